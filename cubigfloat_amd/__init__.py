@@ -1,0 +1,212 @@
+"""Host-side Python mirror of the fpa.h C API (ctypes over the in-tree libfpa.so).
+
+The product is ``libfpa.so`` (C host layer + HIP kernels for gfx950); this module
+only binds it.  It raises at import if the library has not been built -- there is no
+fallback path.
+
+Mirrors the reference's interface (fpa.h:43-63): ``FpaArray`` is an ``fpa_t``
+(count values at one precision in HBM, natural limb layout); ``add/sub/mul/div/sqrt/
+neg`` take (r, a[, b], mode) and return the ``fpa_result`` code like the C functions.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfpa.so")
+
+ROUND_TRUNCATE, ROUND_UP, ROUND_DOWN, ROUND_TIES_TO_EVEN, ROUND_TIES_AWAY = range(5)
+MODES = {"rz": 0, "ru": 1, "rd": 2, "rne": 3, "rna": 4}
+SUCCESS, MATH_ERROR, INVALID_PRECISIONS, INVALID_COUNTS = range(4)
+OPS = {"neg": 0, "add": 1, "sub": 2, "mul": 3, "div": 4, "sqrt": 5}
+PRECISIONS = (256, 512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 16384)
+BIAS = 0x3FFFFFF3
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C cubigfloat_amd")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class FpaStruct(ctypes.Structure):
+    """struct fpa_s (fpa.h): count, precision, gpu_exps, gpu_limbs."""
+    _fields_ = [("count", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("gpu_exps", ctypes.c_void_p), ("gpu_limbs", ctypes.c_void_p)]
+
+
+class FpaArguments(ctypes.Structure):
+    """fpa_arguments (fpa_kernels.h), same fields as the reference's fpa_internal.h:34-46."""
+    _fields_ = [("count", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("set_source_precision", ctypes.c_uint32), ("mode", ctypes.c_int),
+                ("r_exps", ctypes.c_void_p), ("r_limbs", ctypes.c_void_p),
+                ("a_exps", ctypes.c_void_p), ("a_limbs", ctypes.c_void_p),
+                ("b_exps", ctypes.c_void_p), ("b_limbs", ctypes.c_void_p),
+                ("values", ctypes.c_void_p)]
+
+
+lib = ctypes.CDLL(LIB_PATH)
+_P = ctypes.POINTER(FpaStruct)
+for _name in ("fpa_init",):
+    getattr(lib, _name).argtypes = [_P, ctypes.c_uint32, ctypes.c_uint32]
+lib.fpa_clear.argtypes = [_P]
+lib.fpa_copy_array_to_device.argtypes = [_P, _u32p, _u32p]
+lib.fpa_copy_array_to_host.argtypes = [_P, _u32p, _u32p]
+for _name in ("fpa_add", "fpa_sub", "fpa_mul", "fpa_div"):
+    getattr(lib, _name).argtypes = [_P, _P, _P, ctypes.c_int]
+lib.fpa_sqrt.argtypes = [_P, _P, ctypes.c_int]
+lib.fpa_neg.argtypes = [_P, _P]
+lib.fpa_set_ui.argtypes = [_P, ctypes.POINTER(ctypes.c_uint64)]
+lib.fpa_set_si.argtypes = [_P, ctypes.POINTER(ctypes.c_int64)]
+lib.fpa_set_float.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
+lib.fpa_set_double.argtypes = [_P, ctypes.POINTER(ctypes.c_double)]
+lib.fpa_kernel_launch.argtypes = [ctypes.c_int, ctypes.POINTER(FpaArguments), ctypes.c_int, ctypes.c_void_p]
+lib.fpa_kernel_variants.argtypes = [ctypes.c_int, ctypes.c_uint32]
+lib.fpa_kernel_variant_shape.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+lib.fpa_kernel_get_variant.argtypes = [ctypes.c_int, ctypes.c_uint32]
+lib.fpa_kernel_set_variant.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int]
+lib.fpa_kernel_load_tuning.argtypes = [ctypes.c_char_p]
+lib.fpa_set_stream.argtypes = [ctypes.c_void_p]
+lib.fpa_get_stream.restype = ctypes.c_void_p
+lib.fpa_precision_supported.argtypes = [ctypes.c_uint32]
+lib.fpa_scalar_round_inc.argtypes = [ctypes.c_int] + [ctypes.c_uint32] * 4
+lib.fpa_scalar_round_inc.restype = ctypes.c_uint32
+lib.fpa_scalar_recip_3by2.argtypes = [ctypes.c_uint32] * 2
+lib.fpa_scalar_recip_3by2.restype = ctypes.c_uint32
+lib.fpa_scalar_div3by2.argtypes = [ctypes.c_uint32] * 6
+lib.fpa_scalar_div3by2.restype = ctypes.c_uint32
+lib.fpa_scalar_isqrt64.argtypes = [ctypes.c_uint64]
+lib.fpa_scalar_isqrt64.restype = ctypes.c_uint32
+lib.fpa_workload_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
+                                      ctypes.c_uint64, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_u32p)
+
+
+class FpaArray:
+    """An fpa_t: `count` values of `precision` bits in device memory."""
+
+    def __init__(self, count, precision):
+        self.s = FpaStruct()
+        rc = lib.fpa_init(ctypes.byref(self.s), count, precision)
+        if rc != SUCCESS:
+            raise ValueError(f"fpa_init({count}, {precision}) returned {rc}")
+        self.count, self.precision, self.n = count, precision, precision // 32
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+    def upload(self, exps, limbs):
+        exps = np.ascontiguousarray(exps, dtype=np.uint32)
+        limbs = np.ascontiguousarray(limbs, dtype=np.uint32)
+        assert exps.size == self.count and limbs.size == self.count * self.n
+        return lib.fpa_copy_array_to_device(self.ref, _ptr(exps), _ptr(limbs))
+
+    def download(self):
+        e = np.empty(self.count, np.uint32)
+        l = np.empty(self.count * self.n, np.uint32)
+        rc = lib.fpa_copy_array_to_host(self.ref, _ptr(e), _ptr(l))
+        if rc != SUCCESS:
+            raise RuntimeError(f"fpa_copy_array_to_host returned {rc}")
+        return e, l
+
+    def close(self):
+        if self.s is not None:
+            lib.fpa_clear(self.ref)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def add(r, a, b, mode=ROUND_TIES_TO_EVEN):
+    return lib.fpa_add(r.ref, a.ref, b.ref, mode)
+
+
+def sub(r, a, b, mode=ROUND_TIES_TO_EVEN):
+    return lib.fpa_sub(r.ref, a.ref, b.ref, mode)
+
+
+def mul(r, a, b, mode=ROUND_TIES_TO_EVEN):
+    return lib.fpa_mul(r.ref, a.ref, b.ref, mode)
+
+
+def div(r, a, b, mode=ROUND_TIES_TO_EVEN):
+    return lib.fpa_div(r.ref, a.ref, b.ref, mode)
+
+
+def sqrt(r, x, mode=ROUND_TIES_TO_EVEN):
+    return lib.fpa_sqrt(r.ref, x.ref, mode)
+
+
+def neg(r, a):
+    return lib.fpa_neg(r.ref, a.ref)
+
+
+def apply(op, r, a, b=None, mode=ROUND_TIES_TO_EVEN):
+    """Run op by name ('add', 'mul', ...) through the fpa.h entry point."""
+    if op in ("sqrt",):
+        return sqrt(r, a, mode)
+    if op == "neg":
+        return neg(r, a)
+    return {"add": add, "sub": sub, "mul": mul, "div": div}[op](r, a, b, mode)
+
+
+def variants(op, precision):
+    """[(K, L), ...] compiled kernel shapes for (op, precision)."""
+    out = []
+    for v in range(lib.fpa_kernel_variants(OPS[op], precision)):
+        k, l = ctypes.c_int(), ctypes.c_int()
+        lib.fpa_kernel_variant_shape(OPS[op], precision, v, ctypes.byref(k), ctypes.byref(l))
+        out.append((k.value, l.value))
+    return out
+
+
+def set_variant(op, precision, variant):
+    return lib.fpa_kernel_set_variant(OPS[op], precision, variant)
+
+
+def get_variant(op, precision):
+    return lib.fpa_kernel_get_variant(OPS[op], precision)
+
+
+def generate(kind, seed, op, precision, count, first=0, binary=True):
+    """Synthetic operands (fpa_workload_generate): returns (a_exps, a_limbs, b_exps, b_limbs)."""
+    n = precision // 32
+    ae = np.empty(count, np.uint32)
+    al = np.empty(count * n, np.uint32)
+    be = np.empty(count, np.uint32) if binary else None
+    bl = np.empty(count * n, np.uint32) if binary else None
+    rc = lib.fpa_workload_generate(kind, seed, OPS[op], precision, first, count,
+                                   _ptr(ae), _ptr(al), _ptr(be), _ptr(bl))
+    if rc != 0:
+        raise ValueError(f"unknown workload kind {kind}")
+    return ae, al, be, bl
+
+
+def run_op(op, precision, a_exps, a_limbs, b_exps=None, b_limbs=None, mode=ROUND_TIES_TO_EVEN):
+    """Upload, run one op through fpa.h, download.  Returns (r_exps, r_limbs)."""
+    count = len(a_exps)
+    A = FpaArray(count, precision)
+    R = FpaArray(count, precision)
+    B = None
+    try:
+        A.upload(a_exps, a_limbs)
+        if b_exps is not None:
+            B = FpaArray(count, precision)
+            B.upload(b_exps, b_limbs)
+        rc = apply(op, R, A, B, mode)
+        if rc != SUCCESS:
+            raise RuntimeError(f"fpa_{op} returned {rc}")
+        return R.download()
+    finally:
+        for x in (A, B, R):
+            if x is not None:
+                x.close()

@@ -1,0 +1,313 @@
+/*
+ * Wave-group multi-precision integer primitives for CDNA4 (gfx950, wave64).
+ *
+ * A value's n = K*L 32-bit limbs are spread over a group of L consecutive lanes of
+ * one wavefront; group lane j holds limbs [j*K, j*K+K) in registers (natural order,
+ * so a lane's chunk is one contiguous K*4-byte load).  64/L values share a wave.
+ * This replaces the reference's 32-lane warp layout with `size` limbs per lane
+ * (int_warp_math.h:57-90): L and K are template parameters chosen per precision.
+ *
+ * Cross-lane traffic:
+ *   - operand/neighbour exchange: ds_bpermute_b32 (any group size);
+ *   - carry/borrow resolution: two 64-bit ballots per add, resolved with one scalar
+ *     add of the generate/propagate masks (the reference's warp_int_fast_propagate_add,
+ *     int_warp_math.h:402-469, does the same on 32-bit ballots);
+ *   - shifts by arbitrary bit counts: LDS staging, each lane reads the words it needs.
+ * All branch conditions in callers are uniform within a group, so every lane a group
+ * reads from is active.
+ */
+#ifndef FPA_GROUP_H
+#define FPA_GROUP_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fpa_scalar.h"
+
+namespace fpa {
+
+/* Orders LDS accesses of one wavefront: every lane's prior LDS writes are complete and
+ * visible before any lane's later reads.  Groups never span waves, so no workgroup
+ * barrier is needed. */
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t or_all(const uint32_t (&x)[K]) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) r |= x[i];
+  return r;
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t and_all(const uint32_t (&x)[K]) {
+  uint32_t r = 0xFFFFFFFFu;
+#pragma unroll
+  for (int i = 0; i < K; i++) r &= x[i];
+  return r;
+}
+
+/* r = a + b + cin, returns the carry out of the chunk */
+template <int K>
+__device__ __forceinline__ uint32_t chunk_add(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
+                                              uint32_t cin) {
+  uint32_t c = cin;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t t = (uint64_t)a[i] + b[i] + c;
+    r[i] = (uint32_t)t;
+    c = (uint32_t)(t >> 32);
+  }
+  return c;
+}
+
+/* r = a - b - bin, returns the borrow out of the chunk */
+template <int K>
+__device__ __forceinline__ uint32_t chunk_sub(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
+                                              uint32_t bin) {
+  uint32_t c = bin;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t t = (uint64_t)a[i] - b[i] - c;
+    r[i] = (uint32_t)t;
+    c = (uint32_t)(t >> 63);
+  }
+  return c;
+}
+
+/* x += w at word 0, returns the carry out */
+template <int K>
+__device__ __forceinline__ uint32_t chunk_add_word(uint32_t (&x)[K], uint32_t w) {
+  uint32_t c = w;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t t = (uint64_t)x[i] + c;
+    x[i] = (uint32_t)t;
+    c = (uint32_t)(t >> 32);
+  }
+  return c;
+}
+
+/* x -= w at word 0, returns the borrow out */
+template <int K>
+__device__ __forceinline__ uint32_t chunk_sub_word(uint32_t (&x)[K], uint32_t w) {
+  uint32_t c = w;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t t = (uint64_t)x[i] - c;
+    x[i] = (uint32_t)t;
+    c = (uint32_t)(t >> 63);
+  }
+  return c;
+}
+
+template <int K>
+__device__ __forceinline__ void chunk_select(uint32_t (&r)[K], bool c, const uint32_t (&a)[K], const uint32_t (&b)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; i++) r[i] = c ? a[i] : b[i];
+}
+
+/* Coalesced chunk load/store: lane's K limbs are contiguous (p is 4*K-byte aligned). */
+template <int K>
+__device__ __forceinline__ void load_chunk(uint32_t (&x)[K], const uint32_t *p) {
+  if constexpr (K % 4 == 0) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < K / 4; i++) {
+      uint4 t = q[i];
+      x[4 * i] = t.x; x[4 * i + 1] = t.y; x[4 * i + 2] = t.z; x[4 * i + 3] = t.w;
+    }
+  } else if constexpr (K % 2 == 0) {
+    const uint2 *q = reinterpret_cast<const uint2 *>(p);
+#pragma unroll
+    for (int i = 0; i < K / 2; i++) {
+      uint2 t = q[i];
+      x[2 * i] = t.x; x[2 * i + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++) x[i] = p[i];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void store_chunk(uint32_t *p, const uint32_t (&x)[K]) {
+  if constexpr (K % 4 == 0) {
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < K / 4; i++) q[i] = make_uint4(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3]);
+  } else if constexpr (K % 2 == 0) {
+    uint2 *q = reinterpret_cast<uint2 *>(p);
+#pragma unroll
+    for (int i = 0; i < K / 2; i++) q[i] = make_uint2(x[2 * i], x[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++) p[i] = x[i];
+  }
+}
+
+/* LDS chunk store/load (p = group's LDS base + lane offset) */
+template <int K>
+__device__ __forceinline__ void lds_store(uint32_t *p, const uint32_t (&x)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; i++) p[i] = x[i];
+}
+
+template <int K>
+__device__ __forceinline__ void lds_load(uint32_t (&x)[K], const uint32_t *p) {
+#pragma unroll
+  for (int i = 0; i < K; i++) x[i] = p[i];
+}
+
+/* 32 bits of the nw-word number in LDS starting at bit offset pos (may be negative or
+ * beyond the top; missing bits read as zero). */
+__device__ __forceinline__ uint32_t lds_bits(const uint32_t *x, int nw, int pos) {
+  int w = pos >> 5;          /* floor division (arithmetic shift) */
+  int b = pos & 31;
+  uint32_t lo = (w >= 0 && w < nw) ? x[w] : 0u;
+  uint32_t hi = (w + 1 >= 0 && w + 1 < nw) ? x[w + 1] : 0u;
+  return b ? ((lo >> b) | (hi << (32 - b))) : lo;
+}
+
+template <int L>
+struct Group {
+  static_assert(L >= 1 && L <= 64 && (64 % L) == 0, "group size must divide the wave");
+  int lane;   /* 0..63 */
+  int gl;     /* lane within the group */
+  int gbase;  /* first lane of the group */
+
+  __device__ __forceinline__ Group() {
+    lane = (int)__lane_id();
+    gl = lane & (L - 1);
+    gbase = lane - gl;
+  }
+
+  static constexpr uint64_t mask() { return L == 64 ? ~0ull : ((1ull << L) - 1); }
+
+  /* x as held by group lane src (uniform or per-lane src) */
+  __device__ __forceinline__ uint32_t bcast(uint32_t x, int src) const {
+    if constexpr (L == 1) return x;
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((gbase + src) << 2, (int)x);
+  }
+
+  /* x of group lane gl-1; 'fill' in group lane 0 */
+  __device__ __forceinline__ uint32_t from_prev(uint32_t x, uint32_t fill = 0) const {
+    if constexpr (L == 1) return fill;
+    uint32_t y = (uint32_t)__builtin_amdgcn_ds_bpermute((lane - 1) << 2, (int)x);
+    return gl == 0 ? fill : y;
+  }
+
+  /* x of group lane gl+1; 'fill' in the group's top lane */
+  __device__ __forceinline__ uint32_t from_next(uint32_t x, uint32_t fill = 0) const {
+    if constexpr (L == 1) return fill;
+    uint32_t y = (uint32_t)__builtin_amdgcn_ds_bpermute((lane + 1) << 2, (int)x);
+    return gl == L - 1 ? fill : y;
+  }
+
+  /* ballot restricted to the group, bit j = group lane j */
+  __device__ __forceinline__ uint64_t ballot(bool p) const {
+    uint64_t b = __ballot(p);
+    if constexpr (L == 64) return b;
+    return (b >> gbase) & mask();
+  }
+
+  __device__ __forceinline__ bool any(bool p) const { return ballot(p) != 0; }
+
+  /*
+   * Carry-lookahead over the group: lane j generated a carry (g) or passes one through
+   * (p: its chunk is all ones after the local add).  Returns the carry into this lane;
+   * cout = carry out of the top lane.  The carry into lane j is bit j of
+   * (G|P) + G + cin, xor (G|P) xor G.
+   */
+  __device__ __forceinline__ uint32_t resolve(bool g, bool p, uint32_t cin, uint32_t &cout) const {
+    uint64_t G = ballot(g), P = ballot(p);
+    uint64_t X = G | P;
+    if constexpr (L < 64) {
+      uint64_t S = X + G + cin;
+      cout = (uint32_t)((S >> L) & 1u);
+      return (uint32_t)(((S ^ X ^ G) >> gl) & 1u);
+    } else {
+      uint64_t S1 = X + G;
+      uint32_t c1 = S1 < X;
+      uint64_t S = S1 + cin;
+      uint32_t c2 = S < S1;
+      cout = c1 | c2;
+      return (uint32_t)(((S ^ X ^ G) >> gl) & 1u);
+    }
+  }
+
+  /* r = a + b (+ cin into the lowest word of the value); returns the carry out */
+  template <int K>
+  __device__ __forceinline__ uint32_t add(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
+                                          uint32_t cin = 0) const {
+    uint32_t c = chunk_add(r, a, b, 0u);
+    uint32_t cout;
+    uint32_t ci = resolve(c != 0, and_all(r) == 0xFFFFFFFFu, cin, cout);
+    chunk_add_word(r, ci);
+    return cout;
+  }
+
+  /* r = a - b (- bin at the lowest word); returns the borrow out */
+  template <int K>
+  __device__ __forceinline__ uint32_t sub(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
+                                          uint32_t bin = 0) const {
+    uint32_t c = chunk_sub(r, a, b, 0u);
+    uint32_t bout;
+    uint32_t bi = resolve(c != 0, or_all(r) == 0u, bin, bout);
+    chunk_sub_word(r, bi);
+    return bout;
+  }
+
+  /* x += w_lane * 2^(32*K*gl) for a per-lane word w (e.g. carries from the lane below);
+   * returns the carry out of the value */
+  template <int K>
+  __device__ __forceinline__ uint32_t add_lane_words(uint32_t (&x)[K], uint32_t w) const {
+    uint32_t c = chunk_add_word(x, w);
+    uint32_t cout;
+    uint32_t ci = resolve(c != 0, and_all(x) == 0xFFFFFFFFu, 0u, cout);
+    chunk_add_word(x, ci);
+    return cout;
+  }
+
+  /* x -= w_lane * 2^(32*K*gl); returns the borrow out of the value */
+  template <int K>
+  __device__ __forceinline__ uint32_t sub_lane_words(uint32_t (&x)[K], uint32_t w) const {
+    uint32_t c = chunk_sub_word(x, w);
+    uint32_t bout;
+    uint32_t bi = resolve(c != 0, or_all(x) == 0u, 0u, bout);
+    chunk_sub_word(x, bi);
+    return bout;
+  }
+
+  /* limb 'pos' (compile-time) of the group's value */
+  template <int K, int POS>
+  __device__ __forceinline__ uint32_t word(const uint32_t (&x)[K]) const {
+    return bcast(x[POS % K], POS / K);
+  }
+
+  /* number of leading zero bits of the n = K*L word value (32n if zero) */
+  template <int K>
+  __device__ __forceinline__ int clz(const uint32_t (&x)[K]) const {
+    int lz = 0;
+    bool found = false;
+#pragma unroll
+    for (int i = K - 1; i >= 0; i--) {
+      if (!found) {
+        if (x[i]) { lz += __clz(x[i]); found = true; }
+        else lz += 32;
+      }
+    }
+    uint64_t nz = ballot(found);
+    if (nz == 0) return 32 * K * L;
+    int hl = 63 - __clzll(nz);                 /* highest group lane holding a 1 bit */
+    int lzh = (int)bcast((uint32_t)lz, hl);
+    return (L - 1 - hl) * 32 * K + lzh;
+  }
+};
+
+}  // namespace fpa
+
+#endif

@@ -1,0 +1,303 @@
+/*
+ * C host side of fpa.h / fpa_kernels.h: allocation, transfers, argument validation
+ * and dispatch to the HIP kernel layer.
+ *
+ * Replaces the reference's fpa_management.h:1-47 (allocation + host transposes) and
+ * fpa_dispatch.h:1-456 (validation + precision switch + <<<>>>).  Sizes are computed in
+ * size_t (the reference's uint32 products overflow at count*precision >= 2^32), the
+ * device layout equals the host layout (no transpose), and each launch is checked.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/fpa.h"
+#include "../../include/fpa_kernels.h"
+#include "fpa_scalar.h"
+#include "fpa_shapes.h"
+
+/* HIP API failure: print and exit(1), as the reference's $GPU macro (fpa.cu:34) */
+#define FPA_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "\nCall \"%s\" failed from %s, line %d: %s\n", #call, __FILE__,     \
+              __LINE__, hipGetErrorString(e_));                                            \
+      exit(1);                                                                             \
+    }                                                                                      \
+  } while (0)
+
+/* per-op launchers defined in k_<op>.hip */
+int fpa_impl_launch_neg(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_add(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_sub(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_mul(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_div(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_sqrt(const fpa_arguments *a, int K, int L, void *stream);
+
+typedef int (*launch_fn)(const fpa_arguments *, int, int, void *);
+static const launch_fn launchers[FPA_OP_COUNT] = {
+  fpa_impl_launch_neg, fpa_impl_launch_add, fpa_impl_launch_sub,
+  fpa_impl_launch_mul, fpa_impl_launch_div, fpa_impl_launch_sqrt,
+};
+
+/* compiled shape table */
+typedef struct { int n, K, L; } shape_t;
+#define FPA_SHAPE_ENTRY(n_, K_, L_) { n_, K_, L_ },
+static const shape_t shapes[] = { FPA_SHAPES(FPA_SHAPE_ENTRY) };
+#define NSHAPES ((int)(sizeof(shapes) / sizeof(shapes[0])))
+
+/* supported limb counts, in table order */
+static const int prec_n[] = { 8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 512 };
+#define NPREC ((int)(sizeof(prec_n) / sizeof(prec_n[0])))
+
+/* dispatch table: variant index per (op, precision); 0 = first compiled shape */
+static int variant_tab[FPA_OP_COUNT][NPREC];
+static void *cur_stream = NULL;
+
+static int prec_index(uint32_t precision) {
+  if (precision % 32 != 0) return -1;
+  int n = (int)(precision / 32);
+  for (int i = 0; i < NPREC; i++) if (prec_n[i] == n) return i;
+  return -1;
+}
+
+int fpa_precision_supported(uint32_t precision) { return prec_index(precision) >= 0; }
+
+/* variant v of limb count n -> index into shapes[], or -1 */
+static int shape_of(int n, int v) {
+  int k = 0;
+  for (int i = 0; i < NSHAPES; i++) {
+    if (shapes[i].n != n) continue;
+    if (k == v) return i;
+    k++;
+  }
+  return -1;
+}
+
+int fpa_kernel_variants(int op, uint32_t precision) {
+  if (op < 0 || op >= FPA_OP_COUNT || prec_index(precision) < 0) return 0;
+  int n = (int)(precision / 32), k = 0;
+  for (int i = 0; i < NSHAPES; i++) k += shapes[i].n == n;
+  return k;
+}
+
+int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *K, int *L) {
+  if (fpa_kernel_variants(op, precision) <= variant || variant < 0) return -1;
+  int s = shape_of((int)(precision / 32), variant);
+  if (K) *K = shapes[s].K;
+  if (L) *L = shapes[s].L;
+  return 0;
+}
+
+int fpa_kernel_get_variant(int op, uint32_t precision) {
+  int p = prec_index(precision);
+  if (op < 0 || op >= FPA_OP_COUNT || p < 0) return -1;
+  return variant_tab[op][p];
+}
+
+int fpa_kernel_set_variant(int op, uint32_t precision, int variant) {
+  int p = prec_index(precision);
+  if (op < 0 || op >= FPA_OP_COUNT || p < 0) return -1;
+  if (variant < 0 || variant >= fpa_kernel_variants(op, precision)) return -1;
+  variant_tab[op][p] = variant;
+  return 0;
+}
+
+int fpa_kernel_load_tuning(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char line[256];
+  int applied = 0;
+  while (fgets(line, sizeof line, f)) {
+    int op, K, L;
+    unsigned prec;
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%d %u %d %d", &op, &prec, &K, &L) != 4) continue;
+    int nv = fpa_kernel_variants(op, prec);
+    for (int v = 0; v < nv; v++) {
+      int k, l;
+      fpa_kernel_variant_shape(op, prec, v, &k, &l);
+      if (k == K && l == L) { fpa_kernel_set_variant(op, prec, v); applied++; break; }
+    }
+  }
+  fclose(f);
+  return applied;
+}
+
+int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stream) {
+  if (op < 0 || op >= FPA_OP_COUNT || !args) return -1;
+  int s = prec_index(args->precision) < 0 ? -1 : shape_of((int)(args->precision / 32), variant);
+  if (s < 0) return -1;
+  if (args->count == 0) return 0;
+  return launchers[op](args, shapes[s].K, shapes[s].L, stream);
+}
+
+void fpa_set_stream(void *stream) { cur_stream = stream; }
+void *fpa_get_stream(void) { return cur_stream; }
+
+uint32_t fpa_scalar_round_inc(int mode, uint32_t sign, uint32_t lsb, uint32_t rw, uint32_t sticky) {
+  return fpa_round_inc(mode, sign, lsb, rw, sticky);
+}
+uint32_t fpa_scalar_recip_3by2(uint32_t d1, uint32_t d0) { return fpa_recip_3by2(d1, d0); }
+uint32_t fpa_scalar_div3by2(uint32_t u2, uint32_t u1, uint32_t u0, uint32_t d1, uint32_t d0, uint32_t v) {
+  return fpa_div3by2(u2, u1, u0, d1, d0, v);
+}
+uint32_t fpa_scalar_isqrt64(uint64_t u) { return fpa_isqrt64(u); }
+
+/* ------------------------------ fpa.h ------------------------------ */
+
+fpa_result fpa_init(fpa_t fpa, uint32_t count, uint32_t precision) {
+  if (!fpa_precision_supported(precision)) return fpa_invalid_precisions;
+  fpa->count = count;
+  fpa->precision = precision;
+  fpa->gpu_exps = NULL;
+  fpa->gpu_limbs = NULL;
+  size_t words = (size_t)count * (precision / 32);
+  FPA_HIP(hipMalloc((void **)&fpa->gpu_exps, (size_t)count * 4 + 4));
+  FPA_HIP(hipMalloc((void **)&fpa->gpu_limbs, words * 4 + 16));
+  return fpa_success;
+}
+
+fpa_result fpa_clear(fpa_t fpa) {
+  FPA_HIP(hipFree(fpa->gpu_exps));
+  FPA_HIP(hipFree(fpa->gpu_limbs));
+  fpa->gpu_exps = NULL;
+  fpa->gpu_limbs = NULL;
+  return fpa_success;
+}
+
+fpa_result fpa_copy_array_to_device(fpa_t fpa, uint32_t *exps, uint32_t *limbs) {
+  size_t words = (size_t)fpa->count * (fpa->precision / 32);
+  FPA_HIP(hipMemcpy(fpa->gpu_exps, exps, (size_t)fpa->count * 4, hipMemcpyHostToDevice));
+  FPA_HIP(hipMemcpy(fpa->gpu_limbs, limbs, words * 4, hipMemcpyHostToDevice));
+  return fpa_success;
+}
+
+fpa_result fpa_copy_array_to_host(fpa_t fpa, uint32_t *exps, uint32_t *limbs) {
+  size_t words = (size_t)fpa->count * (fpa->precision / 32);
+  if (cur_stream) FPA_HIP(hipStreamSynchronize((hipStream_t)cur_stream));
+  FPA_HIP(hipMemcpy(exps, fpa->gpu_exps, (size_t)fpa->count * 4, hipMemcpyDeviceToHost));
+  FPA_HIP(hipMemcpy(limbs, fpa->gpu_limbs, words * 4, hipMemcpyDeviceToHost));
+  return fpa_success;
+}
+
+static fpa_result run(int op, fpa_ptr r, fpa_ptr a, fpa_ptr b, fpa_rounding_mode mode) {
+  uint32_t count = r->count, precision = r->precision;
+  if (count != a->count || (b && count != b->count)) return fpa_invalid_counts;
+  if (precision != a->precision || (b && precision != b->precision)) return fpa_invalid_precisions;
+  int p = prec_index(precision);
+  if (p < 0) return fpa_invalid_precisions;
+  fpa_arguments args;
+  memset(&args, 0, sizeof args);
+  args.count = count;
+  args.precision = precision;
+  args.mode = mode;
+  args.r_exps = r->gpu_exps;
+  args.r_limbs = r->gpu_limbs;
+  args.a_exps = a->gpu_exps;
+  args.a_limbs = a->gpu_limbs;
+  args.b_exps = b ? b->gpu_exps : a->gpu_exps;
+  args.b_limbs = b ? b->gpu_limbs : a->gpu_limbs;
+  if (count == 0) return fpa_success;
+  int rc = fpa_kernel_launch(op, &args, variant_tab[op][p], cur_stream);
+  if (rc != 0) {
+    fprintf(stderr, "\nkernel launch for op %d, precision %u failed: %d (%s)\n", op, precision, rc,
+            rc > 0 ? hipGetErrorString((hipError_t)rc) : "no such variant");
+    exit(1);
+  }
+  return fpa_success;
+}
+
+fpa_result fpa_neg(fpa_t r, fpa_t a) { return run(FPA_OP_NEG, r, a, NULL, fpa_round_ties_to_even); }
+fpa_result fpa_add(fpa_t r, fpa_t a, fpa_t b, fpa_rounding_mode m) { return run(FPA_OP_ADD, r, a, b, m); }
+fpa_result fpa_sub(fpa_t r, fpa_t a, fpa_t b, fpa_rounding_mode m) { return run(FPA_OP_SUB, r, a, b, m); }
+fpa_result fpa_mul(fpa_t r, fpa_t a, fpa_t b, fpa_rounding_mode m) { return run(FPA_OP_MUL, r, a, b, m); }
+fpa_result fpa_div(fpa_t r, fpa_t a, fpa_t b, fpa_rounding_mode m) { return run(FPA_OP_DIV, r, a, b, m); }
+fpa_result fpa_sqrt(fpa_t r, fpa_t x, fpa_rounding_mode m) { return run(FPA_OP_SQRT, r, x, NULL, m); }
+
+/* ------------------------- conversions (exact) ------------------------- */
+
+/* host staging of |u| * 2^0 with sign into value i of (exps, limbs) */
+static void encode_u64(uint32_t *exp, uint32_t *limbs, int n, uint64_t u, uint32_t sign) {
+  memset(limbs, 0, 4 * (size_t)n);
+  if (u == 0) { *exp = FPA_C_ZERO; return; }
+  int bl = 64 - __builtin_clzll(u);
+  uint64_t m = u << (64 - bl);                     /* top bit set */
+  limbs[n - 1] = (uint32_t)(m >> 32);
+  limbs[n - 2] = (uint32_t)m;
+  *exp = (uint32_t)(2 * (FPA_BIAS_LL + bl) + sign);
+}
+
+/* value = frac * 2^e2 with 53-bit (or 24-bit) integer significand */
+static void encode_fp(uint32_t *exp, uint32_t *limbs, int n, double x) {
+  if (isnan(x)) { memset(limbs, 0, 4 * (size_t)n); *exp = FPA_C_NAN; return; }
+  if (isinf(x)) { memset(limbs, 0, 4 * (size_t)n); *exp = x < 0 ? FPA_C_NEG_INF : FPA_C_POS_INF; return; }
+  if (x == 0) { memset(limbs, 0, 4 * (size_t)n); *exp = FPA_C_ZERO; return; }
+  int e;
+  double f = frexp(fabs(x), &e);                   /* |x| = f * 2^e, f in [1/2, 1) */
+  uint64_t m = (uint64_t)ldexp(f, 64);             /* exact: at most 53 significant bits */
+  memset(limbs, 0, 4 * (size_t)n);
+  limbs[n - 1] = (uint32_t)(m >> 32);
+  limbs[n - 2] = (uint32_t)m;
+  *exp = (uint32_t)(2 * (FPA_BIAS_LL + e) + (x < 0));
+}
+
+static fpa_result upload_staged(fpa_t r, uint32_t *e, uint32_t *l) {
+  fpa_result rc = fpa_copy_array_to_device(r, e, l);
+  free(e);
+  free(l);
+  return rc;
+}
+
+#define STAGE(r)                                                                        \
+  int n_ = (int)((r)->precision / 32);                                                  \
+  uint32_t *se = (uint32_t *)malloc((size_t)(r)->count * 4 + 4);                        \
+  uint32_t *sl = (uint32_t *)malloc((size_t)(r)->count * n_ * 4 + 4);                   \
+  if (!se || !sl) { free(se); free(sl); return fpa_math_error; }
+
+fpa_result fpa_set_ui(fpa_t r, uint64_t *ui) {
+  STAGE(r);
+  for (uint32_t i = 0; i < r->count; i++) encode_u64(&se[i], sl + (size_t)i * n_, n_, ui[i], 0);
+  return upload_staged(r, se, sl);
+}
+
+fpa_result fpa_set_si(fpa_t r, int64_t *si) {
+  STAGE(r);
+  for (uint32_t i = 0; i < r->count; i++) {
+    int64_t v = si[i];
+    uint64_t mag = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
+    encode_u64(&se[i], sl + (size_t)i * n_, n_, mag, v < 0);
+  }
+  return upload_staged(r, se, sl);
+}
+
+fpa_result fpa_set_float(fpa_t r, float *floats) {
+  STAGE(r);
+  for (uint32_t i = 0; i < r->count; i++) encode_fp(&se[i], sl + (size_t)i * n_, n_, (double)floats[i]);
+  return upload_staged(r, se, sl);
+}
+
+fpa_result fpa_set_double(fpa_t r, double *doubles) {
+  STAGE(r);
+  for (uint32_t i = 0; i < r->count; i++) encode_fp(&se[i], sl + (size_t)i * n_, n_, doubles[i]);
+  return upload_staged(r, se, sl);
+}
+
+/* Not on the element-wise arithmetic path; not provided by this build. */
+fpa_result fpa_set(fpa_t r, fpa_t x, fpa_rounding_mode mode) {
+  (void)r; (void)x; (void)mode;
+  return fpa_math_error;
+}
+fpa_result fpa_sgn(int32_t *r, fpa_t x) {
+  (void)r; (void)x;
+  return fpa_math_error;
+}
+fpa_result fpa_cmp(int32_t *r, fpa_t a, fpa_t b) {
+  (void)r;
+  if (a->count != b->count) return fpa_invalid_counts;
+  if (a->precision != b->precision) return fpa_invalid_precisions;
+  return fpa_math_error;
+}

@@ -1,0 +1,66 @@
+/*
+ * Kernel wrapper shared by the per-op compilation units: one wavefront per 64/L
+ * values, 4 waves per 256-thread block, each wave with its own LDS slice (no
+ * workgroup barriers).  Value index v = wave * (64/L) + lane / L; out-of-range groups
+ * read a valid element and store nothing.
+ */
+#ifndef FPA_KERNEL_COMMON_H
+#define FPA_KERNEL_COMMON_H
+
+#include "fpa_ops.h"
+#include "fpa_shapes.h"
+
+namespace fpa {
+
+/* LDS words of scratch per value for each op (neg, add, sub, mul, div, sqrt) */
+template <int OP, int n>
+struct ScratchWords {
+  static constexpr int raw = OP == 0 ? 0 : (OP <= 2 ? n + 1 : (OP == 3 ? n : (OP == 4 ? n + 1 : 2 * n + 1)));
+  /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
+  static constexpr int value = ((raw + 3) / 4) * 4 + 4;
+};
+
+template <int K, int L, int OP>
+__global__ __launch_bounds__(256) void fpa_kernel(fpa_arguments a) {
+  constexpr int n = K * L, VPW = 64 / L;
+  constexpr int SW = ScratchWords<OP, n>::value;
+  __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
+  Group<L> g;
+  const int wib = threadIdx.x >> 6;
+  const int gid = g.lane / L;
+  const uint64_t vv = ((uint64_t)blockIdx.x * 4 + wib) * VPW + gid;
+  Slot s;
+  s.valid = vv < a.count;
+  s.v = s.valid ? (uint32_t)vv : a.count - 1;
+  uint32_t *lds = smem + (wib * VPW + gid) * SW;
+  if constexpr (OP == 0) op_neg<K, L>(a, s, g);
+  else if constexpr (OP == 1) op_addsub<K, L>(a, s, g, lds, false);
+  else if constexpr (OP == 2) op_addsub<K, L>(a, s, g, lds, true);
+  else if constexpr (OP == 3) op_mul<K, L>(a, s, g, lds);
+  else if constexpr (OP == 4) op_div<K, L>(a, s, g, lds);
+  else op_sqrt<K, L>(a, s, g, lds);
+}
+
+template <int K, int L, int OP>
+static int launch_shape(const fpa_arguments &a, hipStream_t st) {
+  constexpr int VPW = 64 / L;
+  if (a.count == 0) return 0;
+  uint64_t waves = ((uint64_t)a.count + VPW - 1) / VPW;
+  uint64_t blocks = (waves + 3) / 4;
+  hipLaunchKernelGGL((fpa_kernel<K, L, OP>), dim3((uint32_t)blocks), dim3(256), 0, st, a);
+  return (int)hipGetLastError();
+}
+
+}  // namespace fpa
+
+/* defines extern "C" int fpa_impl_launch_<name>(const fpa_arguments*, int K, int L, void *stream) */
+#define FPA_DEFINE_LAUNCHER(name, OPV)                                                          \
+  extern "C" int fpa_impl_launch_##name(const fpa_arguments *a, int K, int L, void *stream) { \
+    hipStream_t st = (hipStream_t)stream;                                                       \
+    _Pragma("clang diagnostic push")                                                            \
+    FPA_SHAPES(FPA_LAUNCH_CASE_##name)                                                          \
+    _Pragma("clang diagnostic pop")                                                             \
+    return -1;                                                                                  \
+  }
+
+#endif

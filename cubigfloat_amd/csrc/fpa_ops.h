@@ -1,0 +1,554 @@
+/*
+ * Element-wise correctly rounded operations on the fpa format, one value per group of
+ * L lanes, K limbs per lane (n = K*L limbs, p = 32n bits).
+ *
+ * Per-op structure (reference -> here):
+ *   add/sub  fpa_warp_add_sub.h:41-380 -> op_addsub: align by an LDS funnel shift with a
+ *            round word + sticky below the mantissa, group add/sub with ballot carry
+ *            resolution, renormalise (1 bit, or clz + LDS shift on cancellation), round.
+ *   mul      fpa_warp_mul.h:14-139 -> op_mul: product-scanning by rounds; in round r
+ *            every lane multiplies the broadcast limb chunk r of a (from LDS) by its own
+ *            chunk of b into a 2K-word window; the low window of group lane 0 is final
+ *            and feeds round/sticky; windows then shift down one lane (bpermute).
+ *   div      fpa_warp_div.h:127-201 -> op_div: long division with 32-bit quotient
+ *            digits; digit estimate by the 3-by-2 division of the top three remainder
+ *            words by the top two divisor words (exact up to +2), group
+ *            multiply-subtract, add-back while negative.
+ *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root with 32-bit
+ *            digits in a fixed frame: R <- R*2^32 - d*(2S + d*2^(32 pos)), the digit
+ *            estimated from (R/2)/S's top words, add-back while negative.
+ * Rounding uses fpa_round's decision (fpa_scalar.h) on the exact (round word, sticky)
+ * pair; a carry out of the rounding increment becomes 0x80..0 one binade up.
+ */
+#ifndef FPA_OPS_H
+#define FPA_OPS_H
+
+#include "fpa_group.h"
+#include "../../include/fpa_kernels.h"
+
+namespace fpa {
+
+/* ---- action tables: data restated from the reference (see fpa_oracle.c) ---- */
+__constant__ uint8_t k_add_tab[81] = {
+  FPA_A_ADD,    FPA_A_SUB,    FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_SUB,    FPA_A_ADD,    FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_COPY_B, FPA_A_COPY_B, FPA_C_ZERO,    FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_COPY_B, FPA_A_COPY_B, FPA_C_NEG_TINY, FPA_C_NEG_TINY, FPA_C_NAN,    FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_COPY_B, FPA_A_COPY_B, FPA_C_POS_TINY, FPA_C_NAN,     FPA_C_POS_TINY, FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NAN,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_NAN,     FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_SNAN,
+  FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN,
+};
+__constant__ uint8_t k_sub_tab[81] = {
+  FPA_A_SUB,   FPA_A_ADD,   FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_ADD,   FPA_A_SUB,   FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_A_COPY_A,  FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_NEG_B, FPA_A_NEG_B, FPA_C_ZERO,    FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_NEG_B, FPA_A_NEG_B, FPA_C_NEG_TINY, FPA_C_NAN,    FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_NEG_B, FPA_A_NEG_B, FPA_C_POS_TINY, FPA_C_POS_TINY, FPA_C_NAN,   FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NEG_INF, FPA_C_NAN,  FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_POS_INF, FPA_C_NAN,  FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_SNAN,
+  FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN,
+};
+__constant__ uint8_t k_mul_tab[81] = {
+  FPA_A_MUL,      FPA_A_MUL,      FPA_C_ZERO, FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_MUL,      FPA_A_MUL,      FPA_C_ZERO, FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_ZERO, FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN,     FPA_C_NAN,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_ZERO, FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_NAN,     FPA_C_NAN,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_ZERO, FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NAN,     FPA_C_NAN,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_INF,  FPA_C_POS_INF,  FPA_C_NAN,  FPA_C_NAN,      FPA_C_NAN,      FPA_C_POS_INF, FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_INF,  FPA_C_NEG_INF,  FPA_C_NAN,  FPA_C_NAN,      FPA_C_NAN,      FPA_C_NEG_INF, FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_SNAN,
+  FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN,
+};
+/* classes MPFR represents follow MPFR: x/0, inf/0 -> +-INF; inf/x signed by the
+ * quotient; x/inf -> ZERO.  TINY/SNAN entries are the reference's (fpa_warp_div.h:1-12). */
+__constant__ uint8_t k_div_tab[81] = {
+  FPA_A_DIV,      FPA_A_DIV,      FPA_C_POS_INF, FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_DIV,      FPA_A_DIV,      FPA_C_NEG_INF, FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NAN,     FPA_C_NAN,      FPA_C_NAN,      FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_NAN,     FPA_C_NAN,      FPA_C_NAN,      FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NEG_INF,  FPA_C_POS_INF,  FPA_C_NEG_INF, FPA_C_POS_INF,  FPA_C_NEG_INF,  FPA_C_NAN,      FPA_C_NAN,      FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_POS_INF,  FPA_C_NEG_INF,  FPA_C_POS_INF, FPA_C_NEG_INF,  FPA_C_POS_INF,  FPA_C_NAN,      FPA_C_NAN,      FPA_C_NAN, FPA_C_SNAN,
+  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_NAN,  FPA_C_SNAN,
+  FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN,
+};
+__constant__ uint8_t k_sqrt_tab[9] = { FPA_A_SQRT, FPA_C_NAN, FPA_C_ZERO, FPA_C_NAN, FPA_C_POS_TINY, FPA_C_NAN,
+                                       FPA_C_POS_INF, FPA_C_NAN, FPA_C_SNAN };
+__constant__ uint8_t k_neg_tab[9] = { FPA_A_NEG, FPA_A_NEG, FPA_C_ZERO, FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_POS_INF,
+                                      FPA_C_NEG_INF, FPA_C_NAN, FPA_C_SNAN };
+
+/* exponent word for (biased E, sign) under a [lo, hi] window of E; class code outside */
+__device__ __forceinline__ uint32_t pack_exp(int64_t E, uint32_t sign, int64_t lo, int64_t hi) {
+  if (E < lo) return FPA_C_POS_TINY - sign;
+  if (E > hi) return FPA_C_POS_INF - sign;
+  return (uint32_t)(2 * E + sign);
+}
+
+/* Where a group's value lives: global index and LDS scratch. */
+struct Slot {
+  uint32_t v;       /* value index (clamped to count-1 for reads when invalid) */
+  bool valid;
+};
+
+/* round the group's mantissa x by (round word, sticky); returns 1 if it carried into
+ * a new binade (x is then 0x80..0) */
+template <int K, int L>
+__device__ __forceinline__ uint32_t round_mantissa(const Group<L> &g, uint32_t (&x)[K], int mode, uint32_t sign,
+                                                   uint32_t rw, uint32_t sticky) {
+  uint32_t lsb = g.bcast(x[0], 0) & 1u;
+  uint32_t inc = fpa_round_inc(mode, sign, lsb, rw, sticky);
+  uint32_t cout = g.add_lane_words(x, g.gl == 0 ? inc : 0u);
+  if (cout) {
+#pragma unroll
+    for (int i = 0; i < K; i++) x[i] = 0;
+    if (g.gl == L - 1) x[K - 1] = 0x80000000u;
+  }
+  return cout;
+}
+
+template <int K>
+__device__ __forceinline__ void store_result(const fpa_arguments &a, const Slot &s, int gl, uint32_t rexp,
+                                             const uint32_t (&x)[K]) {
+  if (!s.valid) return;
+  uint32_t z[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) z[i] = rexp > FPA_C_SNAN ? x[i] : 0u;
+  const uint32_t n = a.precision >> 5;
+  store_chunk(a.r_limbs + (size_t)s.v * n + (size_t)gl * K, z);
+  if (gl == 0) a.r_exps[s.v] = rexp;
+}
+
+template <int K>
+__device__ __forceinline__ void store_special(const fpa_arguments &a, const Slot &s, int gl, uint32_t code) {
+  uint32_t z[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) z[i] = 0;
+  store_result(a, s, gl, code, z);
+}
+
+/* ------------------------------- neg ------------------------------- */
+template <int K, int L>
+__device__ void op_neg(const fpa_arguments &a, const Slot &s, const Group<L> &g) {
+  constexpr int n = K * L;
+  uint32_t ea = a.a_exps[s.v];
+  uint32_t act = k_neg_tab[fpa_classify(ea)];
+  uint32_t x[K];
+  load_chunk(x, a.a_limbs + (size_t)s.v * n + (size_t)g.gl * K);
+  store_result(a, s, g.gl, act == FPA_A_NEG ? (ea ^ 1u) : act, x);
+}
+
+/* ----------------------------- add / sub ----------------------------- */
+/* lds: >= n+1 words of this value's scratch */
+template <int K, int L>
+__device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, bool is_sub) {
+  constexpr int n = K * L;
+  const int mode = (int)a.mode;
+  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
+  uint32_t cls = fpa_classify(ea) * 9 + fpa_classify(eb);
+  uint32_t act = is_sub ? k_sub_tab[cls] : k_add_tab[cls];
+  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+
+  if (act != FPA_A_ADD && act != FPA_A_SUB) {
+    uint32_t x[K];
+    if (act == FPA_A_COPY_A) { load_chunk(x, a.a_limbs + off); store_result(a, s, g.gl, ea, x); }
+    else if (act == FPA_A_COPY_B) { load_chunk(x, a.b_limbs + off); store_result(a, s, g.gl, eb, x); }
+    else if (act == FPA_A_NEG_B) { load_chunk(x, a.b_limbs + off); store_result(a, s, g.gl, eb ^ 1u, x); }
+    else store_special<K>(a, s, g.gl, act);
+    return;
+  }
+  if (is_sub) eb ^= 1u;
+
+  uint32_t A[K], B[K];
+  load_chunk(A, a.a_limbs + off);
+  load_chunk(B, a.b_limbs + off);
+  const int64_t Ea = ea >> 1, Eb = eb >> 1;
+  const bool swap = Ea < Eb;
+  uint32_t X[K], Y[K];
+  chunk_select(X, swap, B, A);
+  chunk_select(Y, swap, A, B);
+  const int64_t Ex = swap ? Eb : Ea;
+  const uint32_t sx = swap ? (eb & 1u) : (ea & 1u);
+  const int64_t d = swap ? Eb - Ea : Ea - Eb;
+  uint32_t T[K];
+  int64_t E;
+  uint32_t sign;
+  uint32_t rw = 0, sticky = 0;
+
+  if (act == FPA_A_SUB && d == 0) {
+    /* equal exponents: exact difference, no rounding */
+    uint32_t borrow = g.sub(T, X, Y);
+    sign = sx;
+    if (borrow) {
+      /* |Y| > |X|: negate (two's complement) */
+#pragma unroll
+      for (int i = 0; i < K; i++) T[i] = ~T[i];
+      g.add_lane_words(T, g.gl == 0 ? 1u : 0u);
+      sign = (swap ? ea : eb) & 1u;
+    }
+    int sh = g.clz(T);
+    if (sh == 32 * n) { store_special<K>(a, s, g.gl, FPA_C_ZERO); return; }
+    if (sh) {
+      lds_store(lds + g.gl * K, T);
+      wsync();
+#pragma unroll
+      for (int i = 0; i < K; i++) T[i] = lds_bits(lds, n, 32 * (g.gl * K + i) - sh);
+      wsync();
+    }
+    E = Ex - sh;
+    store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), T);
+    return;
+  }
+
+  /* align Y: Ys = Y >> dd, round word = bits [dd-32, dd), sticky = any bit below */
+  const int dd = d > 32 * n + 32 ? 32 * n + 32 : (int)d;
+  uint32_t Ys[K];
+  lds_store(lds + g.gl * K, Y);
+  wsync();
+#pragma unroll
+  for (int i = 0; i < K; i++) Ys[i] = lds_bits(lds, n, 32 * (g.gl * K + i) + dd);
+  rw = lds_bits(lds, n, dd - 32);
+  wsync();
+  {
+    const int lim = dd - 32;   /* sticky covers bits [0, lim) */
+    uint32_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      int lo_bit = 32 * (g.gl * K + i);
+      if (lo_bit + 32 <= lim) loc |= Y[i];
+      else if (lo_bit < lim) loc |= Y[i] & ((1u << (lim - lo_bit)) - 1u);
+    }
+    sticky = g.any(loc != 0) ? 1u : 0u;
+  }
+  sign = sx;
+
+  if (act == FPA_A_ADD) {
+    uint32_t c = g.add(T, X, Ys);
+    E = Ex;
+    if (c) {
+      /* shift right one bit, the carry becomes the top bit */
+      uint32_t low = g.bcast(T[0], 0) & 1u;
+      uint32_t nxt = g.from_next(T[0], c);
+#pragma unroll
+      for (int i = 0; i < K - 1; i++) T[i] = (T[i] >> 1) | (T[i + 1] << 31);
+      T[K - 1] = (T[K - 1] >> 1) | (nxt << 31);
+      sticky |= rw & 1u;
+      rw = (rw >> 1) | (low << 31);
+      E = Ex + 1;
+    }
+  } else {
+    /* [X,0,0] - [Ys, rw, sticky] */
+    uint32_t b2 = (rw != 0 || sticky) ? 1u : 0u;
+    rw = 0u - rw - sticky;
+    g.sub(T, X, Ys, b2);
+    uint32_t top = g.bcast(T[K - 1], L - 1);
+    E = Ex;
+    if (!(top >> 31)) {
+      /* normalise the (n+1)-word value [T, rw]; more than one bit only when d == 1 (exact) */
+      lds_store(lds + 1 + g.gl * K, T);
+      if (g.gl == 0) lds[0] = rw;
+      wsync();
+      int lz;
+      {
+        int tz = g.clz(T);
+        lz = tz < 32 * n ? tz : 32 * n + __clz(rw);
+      }
+#pragma unroll
+      for (int i = 0; i < K; i++) T[i] = lds_bits(lds, n + 1, 32 * (g.gl * K + i + 1) - lz);
+      rw = lds_bits(lds, n + 1, -lz);
+      wsync();
+      E = Ex - lz;
+    }
+  }
+  if (round_mantissa<K, L>(g, T, mode, sign, rw, sticky)) E += 1;
+  store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), T);
+}
+
+/* -------------------------------- mul -------------------------------- */
+/* lds: n words (the broadcast operand) */
+template <int K, int L>
+__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  constexpr int n = K * L;
+  const int mode = (int)a.mode;
+  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
+  uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  if (act != FPA_A_MUL) { store_special<K>(a, s, g.gl, act); return; }
+  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+  uint32_t b[K], t[K];
+  load_chunk(b, a.b_limbs + off);
+  load_chunk(t, a.a_limbs + off);
+  lds_store(lds + g.gl * K, t);
+  wsync();
+
+  uint32_t W[2 * K];
+#pragma unroll
+  for (int i = 0; i < 2 * K; i++) W[i] = 0;
+  uint32_t cc = 0, sticky = 0, rw = 0;
+  for (int r = 0; r < L; r++) {
+    uint32_t ar[K];
+    lds_load(ar, lds + r * K);
+    /* W += ar * b, row by row; row i's last carry lands in a word no earlier row wrote */
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      uint32_t c = 0;
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        uint64_t tt = (uint64_t)ar[i] * b[j] + W[i + j] + c;
+        W[i + j] = (uint32_t)tt;
+        c = (uint32_t)(tt >> 32);
+      }
+      W[i + K] = c;
+    }
+    /* group lane 0's low window holds final product limbs r*K .. r*K+K-1 */
+    if (r < L - 1) {
+#pragma unroll
+      for (int i = 0; i < K; i++) sticky |= W[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < K - 1; i++) sticky |= W[i];
+      rw = W[K - 1];
+    }
+    /* next window = own high half + next lane's low half + pending carry */
+    uint32_t c = cc;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      uint32_t nx = g.from_next(W[i]);
+      uint64_t tt = (uint64_t)W[K + i] + nx + c;
+      W[i] = (uint32_t)tt;
+      c = (uint32_t)(tt >> 32);
+    }
+    cc = c;
+  }
+  /* high half: window + carries from the lane below */
+  uint32_t H[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) H[i] = W[i];
+  g.add_lane_words(H, g.from_prev(cc));
+  sticky = g.bcast(sticky, 0);
+  rw = g.bcast(rw, 0);
+
+  const uint32_t sign = (ea ^ eb) & 1u;
+  int64_t E = (int64_t)(ea >> 1) + (int64_t)(eb >> 1) - FPA_BIAS_LL;
+  uint32_t top = g.bcast(H[K - 1], L - 1);
+  if (!(top >> 31)) {
+    uint32_t inb = g.from_prev(H[K - 1] >> 31, rw >> 31);
+#pragma unroll
+    for (int i = K - 1; i > 0; i--) H[i] = (H[i] << 1) | (H[i - 1] >> 31);
+    H[0] = (H[0] << 1) | inb;
+    rw <<= 1;
+    E -= 1;
+  }
+  if (round_mantissa<K, L>(g, H, mode, sign, rw, sticky)) E += 1;
+  store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), H);
+}
+
+/* Multiply-subtract step shared by div and sqrt:
+ *   (rtop : R) -= q * (ttop : T), with R, T chunks of n words and rtop signed. */
+template <int K, int L>
+__device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], int64_t &rtop, uint32_t q,
+                                          const uint32_t (&T)[K], uint32_t ttop) {
+  uint32_t c = 0, bw = 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t p = (uint64_t)q * T[i] + c;
+    c = (uint32_t)(p >> 32);
+    uint64_t d = (uint64_t)R[i] - (uint32_t)p - bw;
+    R[i] = (uint32_t)d;
+    bw = (uint32_t)(d >> 63);
+  }
+  uint32_t out = c + bw;                     /* <= 2^32-1: high product word is <= 2^32-2 */
+  uint32_t bout = g.sub_lane_words(R, g.from_prev(out));
+  uint32_t out_top = g.bcast(out, L - 1);
+  rtop = rtop - (int64_t)out_top - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
+}
+
+/* -------------------------------- div -------------------------------- */
+/* lds: n+2 words (quotient digits) */
+template <int K, int L>
+__device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  constexpr int n = K * L;
+  const int mode = (int)a.mode;
+  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
+  uint32_t act = k_div_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  if (act != FPA_A_DIV) { store_special<K>(a, s, g.gl, act); return; }
+  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+  uint32_t A[K], B[K], R[K];
+  load_chunk(A, a.a_limbs + off);
+  load_chunk(B, a.b_limbs + off);
+
+  const uint32_t sign = (ea ^ eb) & 1u;
+  int64_t E = (int64_t)(ea >> 1) - (int64_t)(eb >> 1) + FPA_BIAS_LL;
+  /* leading quotient bit: A >= B */
+  uint32_t below = g.sub(R, A, B);
+  const uint32_t topbit = below ? 0u : 1u;
+  if (below) {
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = A[i];
+  }
+  const uint32_t d1 = g.template word<K, n - 1>(B), d0 = g.template word<K, n - 2>(B);
+  const uint32_t v = fpa_recip_3by2(d1, d0);
+
+  for (int j = 0; j <= n; j++) {
+    /* R <- R * 2^32 */
+    int64_t rtop = g.template word<K, n - 1>(R);
+    uint32_t in = g.from_prev(R[K - 1]);
+#pragma unroll
+    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
+    R[0] = in;
+    const uint32_t u2 = (uint32_t)rtop, u1 = g.template word<K, n - 1>(R), u0 = g.template word<K, n - 2>(R);
+    uint32_t q;
+    if (u2 > d1 || (u2 == d1 && u1 >= d0)) q = 0xFFFFFFFFu;
+    else q = fpa_div3by2(u2, u1, u0, d1, d0, v);
+    msub_step<K, L>(g, R, rtop, q, B, 0u);
+    for (int guard = 0; rtop < 0 && guard < 4; guard++) {
+      uint32_t c = g.add(R, R, B);
+      rtop += c;
+      q -= 1u;
+    }
+    if (g.gl == 0) lds[n - j] = q;
+  }
+  const uint32_t rem_nz = g.any(or_all(R) != 0) ? 1u : 0u;
+  wsync();
+  /* quotient digits Q[0..n] (Q[n] most significant), plus the leading bit */
+  uint32_t M[K];
+  uint32_t rw, sticky;
+  if (topbit) {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      int w = g.gl * K + i;
+      uint32_t hi = (w + 2 <= n) ? lds[w + 2] : 1u;
+      M[i] = (lds[w + 1] >> 1) | (hi << 31);
+    }
+    rw = (lds[1] << 31) | (lds[0] >> 1);
+    sticky = (lds[0] & 1u) | rem_nz;
+    E += 1;
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++) M[i] = lds[g.gl * K + i + 1];
+    rw = lds[0];
+    sticky = rem_nz;
+  }
+  wsync();
+  if (round_mantissa<K, L>(g, M, mode, sign, rw, sticky)) E += 1;
+  store_result(a, s, g.gl, pack_exp(E, sign, 10, 0x7FFFFFFEll), M);
+}
+
+/* -------------------------------- sqrt -------------------------------- */
+/* lds: 2n+2 words: T = 2S (+ current digit) at [0, n+1), root digits at [n+1, 2n+1) */
+template <int K, int L>
+__device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  constexpr int n = K * L;
+  const int mode = (int)a.mode;
+  uint32_t ea = a.a_exps[s.v];
+  uint32_t act = k_sqrt_tab[fpa_classify(ea)];
+  if (act != FPA_A_SQRT) { store_special<K>(a, s, g.gl, act); return; }
+  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+  uint32_t *Tl = lds, *Sl = lds + n + 1;
+  uint32_t A[K], R[K];
+  load_chunk(A, a.a_limbs + off);
+  int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
+  uint32_t shift_in = 0;
+  if (e & 1) {
+    /* odd exponent: radicand A/2, the dropped bit enters with the first shift */
+    shift_in = (g.bcast(A[0], 0) & 1u) << 31;
+    uint32_t nx = g.from_next(A[0]);
+#pragma unroll
+    for (int i = 0; i < K - 1; i++) R[i] = (A[i] >> 1) | (A[i + 1] << 31);
+    R[K - 1] = (A[K - 1] >> 1) | (nx << 31);
+    e += 1;
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = A[i];
+  }
+  const int64_t er = e / 2;
+#pragma unroll
+  for (int i = 0; i < K; i++) Tl[g.gl * K + i] = 0;
+  if (g.gl == 0) Tl[n] = 0;
+
+  /* first digit: integer root of the top two radicand words */
+  int64_t rtop;
+  uint32_t dtop;
+  {
+    uint32_t rt = g.template word<K, n - 1>(R);
+    uint32_t in = g.from_prev(R[K - 1], shift_in);
+#pragma unroll
+    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
+    R[0] = in;
+    uint64_t u = ((uint64_t)rt << 32) | g.template word<K, n - 1>(R);
+    uint32_t d1 = fpa_isqrt64(u);
+    uint64_t rem = u - (uint64_t)d1 * d1;
+    if (g.gl == L - 1) R[K - 1] = (uint32_t)rem;
+    rtop = (int64_t)(rem >> 32);
+    dtop = d1 >> 31;
+    wsync();
+    if (g.gl == 0) { Tl[n - 1] = d1 << 1; Sl[n - 1] = d1; }
+    wsync();
+  }
+  uint32_t v = 0;
+  for (int pos = n - 2; pos >= 0; pos--) {
+    /* R <- R * 2^32 */
+    uint32_t rt = g.template word<K, n - 1>(R);
+    uint32_t in = g.from_prev(R[K - 1]);
+#pragma unroll
+    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
+    R[0] = in;
+    rtop = (rtop << 32) | rt;
+    /* estimate: floor((R/2) / S) from the top words, S = T/2 */
+    const uint32_t r1 = g.template word<K, n - 1>(R), r2 = g.template word<K, n - 2>(R);
+    const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
+    const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
+    const uint32_t u0 = (r1 << 31) | (r2 >> 1);
+    const uint32_t t1 = Tl[n - 1], t2 = Tl[n - 2];
+    const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
+    if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
+    uint32_t d;
+    if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
+    else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
+    /* R -= d * (T + d*2^(32 pos)) */
+    if (g.gl == 0) Tl[pos] = d;
+    wsync();
+    uint32_t Tr[K];
+    lds_load(Tr, Tl + g.gl * K);
+    msub_step<K, L>(g, R, rtop, d, Tr, dtop);
+    for (int guard = 0; rtop < 0 && guard < 6; guard++) {
+      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (T + (d-1)*2^(32pos)) */
+      uint32_t c = g.add(R, R, Tr);
+      rtop += (int64_t)c + dtop;
+      d -= 1u;
+      wsync();
+      if (g.gl == 0) Tl[pos] = d;
+      wsync();
+      lds_load(Tr, Tl + g.gl * K);
+      c = g.add(R, R, Tr);
+      rtop += (int64_t)c + dtop;
+    }
+    wsync();
+    /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */
+    if (g.gl == 0) {
+      Tl[pos] = d << 1;
+      Tl[pos + 1] |= d >> 31;
+      Sl[pos] = d;
+    }
+    wsync();
+  }
+  /* root S (n words), remainder (rtop : R) = X - S^2 */
+  uint32_t S[K], D[K];
+  lds_load(S, Sl + g.gl * K);
+  const uint32_t rem_nz = (rtop != 0 || g.any(or_all(R) != 0)) ? 1u : 0u;
+  uint32_t gt;
+  if (rtop > 0) gt = 1;
+  else {
+    uint32_t bo = g.sub(D, R, S);
+    gt = (!bo && g.any(or_all(D) != 0)) ? 1u : 0u;
+  }
+  wsync();
+  int64_t E = er + FPA_BIAS_LL;
+  if (round_mantissa<K, L>(g, S, mode, 0u, gt ? 0x80000000u : 0u, rem_nz)) E += 1;
+  store_result(a, s, g.gl, pack_exp(E, 0u, 0, 0x7FFFFFFFll), S);
+}
+
+}  // namespace fpa
+
+#endif

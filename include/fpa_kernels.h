@@ -1,0 +1,83 @@
+/*
+ * fpa_kernels.h -- C ABI of the HIP kernel layer under fpa.h.
+ *
+ * The reference's kernel ABI is one __global__ per op and size, all taking an
+ * fpa_arguments struct by value (fpa_kernels.h:1-191, fpa_internal.h:34-46), launched
+ * by a precision switch in fpa_dispatch.h.  Here a single launcher takes the same
+ * descriptor plus a kernel *variant* -- a (limbs-per-lane K, lanes-per-value L) shape
+ * with K*L = precision/32 -- and a HIP stream.  Plain pointers and sizes only.
+ */
+#ifndef FPA_KERNELS_H
+#define FPA_KERNELS_H
+
+#include <stdint.h>
+#include "fpa.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same fields and meaning as the reference's fpa_arguments (fpa_internal.h:34-46). */
+typedef struct {
+  uint32_t           count;
+  uint32_t           precision;
+  uint32_t           set_source_precision;
+  fpa_rounding_mode  mode;
+  uint32_t          *r_exps;
+  uint32_t          *r_limbs;
+  uint32_t          *a_exps;
+  uint32_t          *a_limbs;
+  uint32_t          *b_exps;
+  uint32_t          *b_limbs;
+  void              *values;
+} fpa_arguments;
+
+typedef enum {
+  FPA_OP_NEG = 0,
+  FPA_OP_ADD = 1,
+  FPA_OP_SUB = 2,
+  FPA_OP_MUL = 3,
+  FPA_OP_DIV = 4,
+  FPA_OP_SQRT = 5,
+  FPA_OP_COUNT = 6
+} fpa_op;
+
+/* Launch op over args->count elements with kernel variant `variant` (index into
+ * fpa_kernel_variants) on `stream` (a hipStream_t, NULL = null stream).
+ * Returns 0 on success, otherwise a hipError_t value (or -1 for an unknown
+ * op/precision/variant).  count == 0 launches nothing. */
+int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stream);
+
+/* Number of compiled variants for (op, precision); 0 if the precision is unsupported. */
+int fpa_kernel_variants(int op, uint32_t precision);
+
+/* Shape of a variant: limbs per lane, lanes per value. Returns 0, or -1 if invalid. */
+int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *limbs_per_lane, int *lanes_per_value);
+
+/* Variant the fpa.h dispatcher uses for (op, precision), and an override of it
+ * (the autotuner writes these).  Returns -1 for invalid arguments. */
+int fpa_kernel_get_variant(int op, uint32_t precision);
+int fpa_kernel_set_variant(int op, uint32_t precision, int variant);
+
+/* Load "op precision K L" lines (as written by the autotuner) into the dispatch table.
+ * Returns the number of entries applied, or -1 if the file cannot be read. */
+int fpa_kernel_load_tuning(const char *path);
+
+/* Stream used by the fpa.h functions (a hipStream_t; NULL = null stream). */
+void fpa_set_stream(void *stream);
+void *fpa_get_stream(void);
+
+/* 1 if precision (bits) is supported */
+int fpa_precision_supported(uint32_t precision);
+
+/* Scalar helpers exported for host-side tests (same code as the kernels use). */
+uint32_t fpa_scalar_round_inc(int mode, uint32_t sign, uint32_t lsb, uint32_t round_word, uint32_t sticky);
+uint32_t fpa_scalar_recip_3by2(uint32_t d1, uint32_t d0);
+uint32_t fpa_scalar_div3by2(uint32_t u2, uint32_t u1, uint32_t u0, uint32_t d1, uint32_t d0, uint32_t v);
+uint32_t fpa_scalar_isqrt64(uint64_t u);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,498 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Exact-integer CPU restatement of cuBigFloat's element-wise operations
+ * (neg/add/sub/mul/div/sqrt) on the fpa number format.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this code,
+ * and only as the checker; the product (cubigfloat_amd/) never calls it.
+ *
+ * Number format (fpa_internal.h:1-17, tests.c:114-159):
+ *   exponent word  <= 8 : class code (2 ZERO, 3 NEG_TINY, 4 POS_TINY, 5 NEG_INF,
+ *                          6 POS_INF, 7 NAN, 8 SNAN)
+ *   exponent word  >= 10: bit 0 = sign, E = word>>1, value = (-1)^s * M * 2^(E-BIAS-p),
+ *                          M = p-bit integer with its top bit set, p = 32*n,
+ *                          little-endian 32-bit limbs.
+ *
+ * Every function computes the exact result with big-integer arithmetic and rounds
+ * it once, so finite results equal MPFR's correctly rounded ones.  Where the
+ * reference's GPU code departs from correct rounding or leaves an action
+ * unhandled, the oracle follows MPFR (the reference's own gold, tests.c:756-918):
+ *   - sub with a==0: the reference table says NEG_B (fpa_warp_add_sub.h:32-34) but its
+ *     switch has no NEG_B case (fpa_warp_add_sub.h:357-379); the oracle returns -b.
+ *   - div special classes that MPFR can represent (values, zero, +-inf, nan) follow
+ *     MPFR: x/0 and inf/0 -> +-INF (reference table: NAN, fpa_warp_div.h:3-9),
+ *     inf/x -> INF with the quotient's sign (the reference's inf rows,
+ *     fpa_warp_div.h:8-9, carry the opposite sign), x/inf -> ZERO (reference: TINY).
+ *   - mul and sub-diff "critical" all-ones paths (fpa_warp_mul.h:99-113,
+ *     fpa_warp_add_sub.h:187-200) round at the unshifted position; the oracle rounds
+ *     the normalised value (e.g. 0x8000..0 * 0xFFFF..F is exact).
+ *   - sqrt round-up past all-ones (directed modes) raises the exponent by one binade;
+ *     the reference adds 1 to the exponent word (fpa_warp_sqrt.h:238-243).
+ *
+ * Range rules restated from the reference's exponent checks:
+ *   add/sub/mul : E < 5 -> +-TINY, E > 0x7FFFFFFF -> +-INF
+ *                 (fpa_warp_add_sub.h:118, 226, 272; fpa_warp_mul.h:125-128)
+ *   div         : E < 10 -> +-TINY, E >= 0x7FFFFFFF -> +-INF (fpa_warp_div.h:190-193)
+ *   sqrt        : no range check (fpa_warp_sqrt.h:247-250)
+ * Special-value action tables are restated as data from fpa_warp_add_sub.h:3-39,
+ * fpa_warp_mul.h:1-12, fpa_warp_div.h:1-12, fpa_warp_sqrt.h:1-11.
+ * Rounding decision restated from fpa_round (fpa_internal.h:62-88).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_BIAS 0x3FFFFFF3ll
+
+enum { C_POSV = 0, C_NEGV = 1, C_ZERO = 2, C_NTINY = 3, C_PTINY = 4, C_NINF = 5, C_PINF = 6, C_NAN = 7, C_SNAN = 8 };
+enum { A_MUL = 16, A_DIV = 17, A_SQRT = 18, A_NEG = 19, A_ADD = 20, A_SUB = 21, A_COPY_A = 22, A_COPY_B = 23, A_NEG_B = 24 };
+enum { M_RZ = 0, M_RU = 1, M_RD = 2, M_RNE = 3, M_RNA = 4 };
+
+/* ---- action tables (data restated from the reference, see header) ---- */
+static const uint8_t neg_tab[9] = { A_NEG, A_NEG, C_ZERO, C_PTINY, C_NTINY, C_PINF, C_NINF, C_NAN, C_SNAN };
+
+#define V C_POSV
+static const uint8_t add_tab[81] = {
+  A_ADD,    A_SUB,    A_COPY_A, A_COPY_A, A_COPY_A, C_NINF, C_PINF, C_NAN, C_SNAN,
+  A_SUB,    A_ADD,    A_COPY_A, A_COPY_A, A_COPY_A, C_NINF, C_PINF, C_NAN, C_SNAN,
+  A_COPY_B, A_COPY_B, C_ZERO,   C_NTINY,  C_PTINY,  C_NINF, C_PINF, C_NAN, C_SNAN,
+  A_COPY_B, A_COPY_B, C_NTINY,  C_NTINY,  C_NAN,    C_NINF, C_PINF, C_NAN, C_SNAN,
+  A_COPY_B, A_COPY_B, C_PTINY,  C_NAN,    C_PTINY,  C_NINF, C_PINF, C_NAN, C_SNAN,
+  C_NINF,   C_NINF,   C_NINF,   C_NINF,   C_NINF,   C_NINF, C_NAN,  C_NAN, C_SNAN,
+  C_PINF,   C_PINF,   C_PINF,   C_PINF,   C_PINF,   C_NAN,  C_PINF, C_NAN, C_SNAN,
+  C_NAN,    C_NAN,    C_NAN,    C_NAN,    C_NAN,    C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN, C_SNAN, C_SNAN, C_SNAN,
+};
+static const uint8_t sub_tab[81] = {
+  A_SUB,    A_ADD,    A_COPY_A, A_COPY_A, A_COPY_A, C_PINF, C_NINF, C_NAN, C_SNAN,
+  A_ADD,    A_SUB,    A_COPY_A, A_COPY_A, A_COPY_A, C_PINF, C_NINF, C_NAN, C_SNAN,
+  A_NEG_B,  A_NEG_B,  C_ZERO,   C_PTINY,  C_NTINY,  C_PINF, C_NINF, C_NAN, C_SNAN,
+  A_NEG_B,  A_NEG_B,  C_NTINY,  C_NAN,    C_NINF,   C_PINF, C_NINF, C_NAN, C_SNAN,
+  A_NEG_B,  A_NEG_B,  C_PTINY,  C_PTINY,  C_NAN,    C_PINF, C_NINF, C_NAN, C_SNAN,
+  C_NINF,   C_NINF,   C_NINF,   C_NINF,   C_NINF,   C_NAN,  C_NINF, C_NAN, C_SNAN,
+  C_PINF,   C_PINF,   C_PINF,   C_PINF,   C_PINF,   C_PINF, C_NAN,  C_NAN, C_SNAN,
+  C_NAN,    C_NAN,    C_NAN,    C_NAN,    C_NAN,    C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN,   C_SNAN, C_SNAN, C_SNAN, C_SNAN,
+};
+static const uint8_t mul_tab[81] = {
+  A_MUL,   A_MUL,   C_ZERO, C_NTINY, C_PTINY, C_NINF, C_PINF, C_NAN, C_SNAN,
+  A_MUL,   A_MUL,   C_ZERO, C_PTINY, C_NTINY, C_PINF, C_NINF, C_NAN, C_SNAN,
+  C_ZERO,  C_ZERO,  C_ZERO, C_ZERO,  C_ZERO,  C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_NTINY, C_PTINY, C_ZERO, C_PTINY, C_NTINY, C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_PTINY, C_NTINY, C_ZERO, C_NTINY, C_PTINY, C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_NINF,  C_PINF,  C_NAN,  C_NAN,   C_NAN,   C_PINF, C_NINF, C_NAN, C_SNAN,
+  C_PINF,  C_NINF,  C_NAN,  C_NAN,   C_NAN,   C_NINF, C_PINF, C_NAN, C_SNAN,
+  C_NAN,   C_NAN,   C_NAN,  C_NAN,   C_NAN,   C_NAN,  C_NAN,  C_NAN, C_SNAN,
+  C_SNAN,  C_SNAN,  C_SNAN, C_SNAN,  C_SNAN,  C_SNAN, C_SNAN, C_SNAN, C_SNAN,
+};
+/* Rows/columns MPFR can represent (values, zero, +-inf, nan) follow MPFR:
+ * x/0 -> +-INF (reference: NAN), inf/0 -> +-INF (reference: NAN), inf/x has the sign
+ * of the quotient (the reference's inf rows have it flipped), x/inf -> ZERO (MPFR +-0;
+ * reference: TINY).  Entries with a TINY or SNAN operand are the reference's. */
+static const uint8_t div_tab[81] = {
+  A_DIV,   A_DIV,   C_PINF, C_NTINY, C_PTINY, C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
+  A_DIV,   A_DIV,   C_NINF, C_PTINY, C_NTINY, C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
+  C_ZERO,  C_ZERO,  C_NAN,  C_ZERO,  C_ZERO,  C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
+  C_NTINY, C_PTINY, C_NAN,  C_NAN,   C_NAN,   C_PTINY, C_NTINY, C_NAN, C_SNAN,
+  C_PTINY, C_NTINY, C_NAN,  C_NAN,   C_NAN,   C_NTINY, C_PTINY, C_NAN, C_SNAN,
+  C_NINF,  C_PINF,  C_NINF, C_PINF,  C_NINF,  C_NAN,   C_NAN,   C_NAN, C_SNAN,
+  C_PINF,  C_NINF,  C_PINF, C_NINF,  C_PINF,  C_NAN,   C_NAN,   C_NAN, C_SNAN,
+  C_NAN,   C_NAN,   C_NAN,  C_NAN,   C_NAN,   C_NAN,   C_NAN,   C_NAN, C_SNAN,
+  C_SNAN,  C_SNAN,  C_SNAN, C_SNAN,  C_SNAN,  C_SNAN,  C_SNAN,  C_SNAN, C_SNAN,
+};
+#undef V
+static const uint8_t sqrt_tab[9] = { A_SQRT, C_NAN, C_ZERO, C_NAN, C_PTINY, C_NAN, C_PINF, C_NAN, C_SNAN };
+
+static uint32_t classify(uint32_t w) { return w <= C_SNAN ? w : (w & 1u); }
+
+/* expose the tables to the tests (restated data is checked against the reference text) */
+int orc_action(int op, uint32_t a_exp, uint32_t b_exp) {
+  uint32_t ca = classify(a_exp), cb = classify(b_exp);
+  switch (op) {
+    case 0: return neg_tab[ca];
+    case 1: return add_tab[ca * 9 + cb];
+    case 2: return sub_tab[ca * 9 + cb];
+    case 3: return mul_tab[ca * 9 + cb];
+    case 4: return div_tab[ca * 9 + cb];
+    case 5: return sqrt_tab[ca];
+  }
+  return -1;
+}
+
+/* ---------------------------- big integers ---------------------------- */
+static int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+
+static int bn_len(const uint32_t *x, int n) { while (n > 0 && x[n - 1] == 0) n--; return n; }
+
+static int64_t bn_bitlen(const uint32_t *x, int n) {
+  n = bn_len(x, n);
+  if (n == 0) return 0;
+  return (int64_t)32 * n - clz32(x[n - 1]);
+}
+
+static int bn_bit(const uint32_t *x, int n, int64_t i) {
+  if (i < 0 || i >= (int64_t)32 * n) return 0;
+  return (x[i >> 5] >> (i & 31)) & 1;
+}
+
+/* any bit of x below position i set? */
+static int bn_any_below(const uint32_t *x, int n, int64_t i) {
+  if (i <= 0) return 0;
+  if (i >= (int64_t)32 * n) return bn_len(x, n) != 0;
+  int64_t w = i >> 5;
+  for (int64_t k = 0; k < w; k++) if (x[k]) return 1;
+  int b = (int)(i & 31);
+  return b && (x[w] & ((1u << b) - 1)) != 0;
+}
+
+/* r[0..nr) = x >> s (s >= 0), bits above x are zero */
+static void bn_shr(uint32_t *r, int nr, const uint32_t *x, int nx, int64_t s) {
+  for (int i = 0; i < nr; i++) {
+    int64_t bit = (int64_t)i * 32 + s;
+    int64_t w = bit >> 5;
+    int b = (int)(bit & 31);
+    uint32_t lo = (w < nx) ? x[w] : 0, hi = (w + 1 < nx) ? x[w + 1] : 0;
+    r[i] = b ? ((lo >> b) | (hi << (32 - b))) : lo;
+  }
+}
+
+/* r[0..nr) = x << s (s >= 0), truncated to nr words */
+static void bn_shl(uint32_t *r, int nr, const uint32_t *x, int nx, int64_t s) {
+  for (int i = 0; i < nr; i++) {
+    int64_t bit = (int64_t)i * 32 - s;
+    int64_t w = bit >= 0 ? (bit >> 5) : -((-bit + 31) >> 5);
+    int b = (int)(bit - w * 32);
+    uint32_t lo = (w >= 0 && w < nx) ? x[w] : 0, hi = (w + 1 >= 0 && w + 1 < nx) ? x[w + 1] : 0;
+    r[i] = b ? ((lo >> b) | (hi << (32 - b))) : lo;
+  }
+}
+
+static int bn_cmp(const uint32_t *a, const uint32_t *b, int n) {
+  for (int i = n - 1; i >= 0; i--) if (a[i] != b[i]) return a[i] > b[i] ? 1 : -1;
+  return 0;
+}
+
+/* r = a + b, returns carry */
+static uint32_t bn_add(uint32_t *r, const uint32_t *a, const uint32_t *b, int n) {
+  uint64_t c = 0;
+  for (int i = 0; i < n; i++) { c += (uint64_t)a[i] + b[i]; r[i] = (uint32_t)c; c >>= 32; }
+  return (uint32_t)c;
+}
+
+/* r = a - b, returns borrow */
+static uint32_t bn_sub(uint32_t *r, const uint32_t *a, const uint32_t *b, int n) {
+  int64_t c = 0;
+  for (int i = 0; i < n; i++) { c += (int64_t)a[i] - b[i]; r[i] = (uint32_t)c; c >>= 32; }
+  return (uint32_t)(c != 0);
+}
+
+/* r[0..na+nb) = a * b */
+static void bn_mul(uint32_t *r, const uint32_t *a, int na, const uint32_t *b, int nb) {
+  memset(r, 0, sizeof(uint32_t) * (na + nb));
+  for (int i = 0; i < na; i++) {
+    uint64_t c = 0;
+    for (int j = 0; j < nb; j++) {
+      uint64_t t = (uint64_t)a[i] * b[j] + r[i + j] + c;
+      r[i + j] = (uint32_t)t;
+      c = t >> 32;
+    }
+    r[i + nb] = (uint32_t)c;
+  }
+}
+
+/*
+ * Knuth algorithm D: q[0..nu-nv] = floor(u / v), rem[0..nv) = u mod v.
+ * nu >= nv >= 1, v[nv-1] != 0.
+ */
+static void bn_divmod(uint32_t *q, uint32_t *rem, const uint32_t *u, int nu, const uint32_t *v, int nv) {
+  if (nv == 1) {
+    uint64_t r = 0;
+    for (int i = nu - 1; i >= 0; i--) {
+      uint64_t cur = (r << 32) | u[i];
+      q[i] = (uint32_t)(cur / v[0]);
+      r = cur % v[0];
+    }
+    rem[0] = (uint32_t)r;
+    return;
+  }
+  int s = clz32(v[nv - 1]);
+  uint32_t *vn = (uint32_t *)malloc(sizeof(uint32_t) * nv);
+  uint32_t *un = (uint32_t *)malloc(sizeof(uint32_t) * (nu + 1));
+  for (int i = nv - 1; i > 0; i--) vn[i] = (v[i] << s) | (s ? v[i - 1] >> (32 - s) : 0);
+  vn[0] = v[0] << s;
+  un[nu] = s ? u[nu - 1] >> (32 - s) : 0;
+  for (int i = nu - 1; i > 0; i--) un[i] = (u[i] << s) | (s ? u[i - 1] >> (32 - s) : 0);
+  un[0] = u[0] << s;
+  for (int j = nu - nv; j >= 0; j--) {
+    uint64_t num = ((uint64_t)un[j + nv] << 32) | un[j + nv - 1];
+    uint64_t qh = num / vn[nv - 1], rh = num % vn[nv - 1];
+    while (qh >> 32 || qh * vn[nv - 2] > ((rh << 32) | un[j + nv - 2])) {
+      qh--;
+      rh += vn[nv - 1];
+      if (rh >> 32) break;
+    }
+    /* un[j..j+nv] -= qh * vn */
+    int64_t borrow = 0;
+    uint64_t carry = 0;
+    for (int i = 0; i < nv; i++) {
+      uint64_t p = qh * vn[i] + carry;
+      carry = p >> 32;
+      int64_t t = (int64_t)un[i + j] - (int64_t)(uint32_t)p + borrow;
+      un[i + j] = (uint32_t)t;
+      borrow = t >> 32;
+    }
+    int64_t t = (int64_t)un[j + nv] - (int64_t)carry + borrow;
+    un[j + nv] = (uint32_t)t;
+    if (t < 0) { /* add back */
+      qh--;
+      uint64_t c = 0;
+      for (int i = 0; i < nv; i++) { c += (uint64_t)un[i + j] + vn[i]; un[i + j] = (uint32_t)c; c >>= 32; }
+      un[j + nv] += (uint32_t)c;
+    }
+    q[j] = (uint32_t)qh;
+  }
+  for (int i = 0; i < nv; i++) rem[i] = (un[i] >> s) | (s ? un[i + 1] << (32 - s) : 0);
+  free(vn);
+  free(un);
+}
+
+/* r[0..nx/2+1) = floor(sqrt(x)), returns 1 if x is not a perfect square. Newton from above. */
+static int bn_isqrt(uint32_t *r, const uint32_t *x, int nx) {
+  int nr = nx / 2 + 1;
+  int64_t bl = bn_bitlen(x, nx);
+  uint32_t *cur = (uint32_t *)calloc(nr + 1, 4), *nxt = (uint32_t *)calloc(nr + 1, 4);
+  uint32_t *qq = (uint32_t *)calloc(nx + 1, 4), *rr = (uint32_t *)calloc(nr + 2, 4);
+  if (bl == 0) { memset(r, 0, 4 * nr); free(cur); free(nxt); free(qq); free(rr); return 0; }
+  int64_t e = (bl + 1) / 2;  /* 2^e >= sqrt(x) */
+  cur[e >> 5] |= 1u << (e & 31);
+  for (;;) {
+    int lc = bn_len(cur, nr + 1);
+    /* nxt = (cur + x / cur) / 2 */
+    memset(qq, 0, 4 * (nx + 1));
+    bn_divmod(qq, rr, x, nx, cur, lc);
+    uint32_t *sum = (uint32_t *)calloc(nx + 2, 4);
+    uint32_t *cc = (uint32_t *)calloc(nx + 2, 4);
+    memcpy(cc, cur, 4 * lc);
+    uint32_t c = bn_add(sum, qq, cc, nx + 1);
+    sum[nx + 1] = c;
+    memset(nxt, 0, 4 * (nr + 1));
+    bn_shr(nxt, nr + 1, sum, nx + 2, 1);
+    free(sum); free(cc);
+    if (bn_cmp(nxt, cur, nr + 1) >= 0) break;
+    memcpy(cur, nxt, 4 * (nr + 1));
+  }
+  memcpy(r, cur, 4 * nr);
+  /* inexact iff cur*cur != x */
+  uint32_t *sq = (uint32_t *)calloc(2 * (nr + 1), 4);
+  bn_mul(sq, cur, nr, cur, nr);
+  int inexact = 0;
+  for (int i = 0; i < 2 * nr; i++) {
+    uint32_t xi = i < nx ? x[i] : 0;
+    if (sq[i] != xi) { inexact = 1; break; }
+  }
+  free(sq); free(cur); free(nxt); free(qq); free(rr);
+  return inexact;
+}
+
+/* ------------------------------ rounding ------------------------------ */
+
+/* fpa_round (fpa_internal.h:62-88) expressed with the first dropped bit and
+ * the OR of the rest. Returns 1 to increment the magnitude. */
+int orc_round_decision(int mode, int sign, int lsb, int round_bit, int sticky) {
+  switch (mode) {
+    case M_RZ: return 0;
+    case M_RU: return !sign && (round_bit || sticky);
+    case M_RD: return sign && (round_bit || sticky);
+    case M_RNA: return round_bit;
+    case M_RNE: return round_bit && (sticky || lsb);
+  }
+  return 0;
+}
+
+/*
+ * Round |value| = (X + frac) * 2^scale, 0 <= frac < 1 with frac > 0 iff sticky_in,
+ * to p = 32n bits.  X must be nonzero.  Writes the mantissa, returns the MPFR-style
+ * exponent e (value = 0.M * 2^e).
+ */
+static int64_t round_to(uint32_t *m, int n, const uint32_t *X, int nx, int64_t scale, int sticky_in, int sign, int mode) {
+  int64_t p = (int64_t)32 * n;
+  int64_t L = bn_bitlen(X, nx);
+  int64_t e = L + scale;
+  int rbit, sticky;
+  if (L > p) {
+    int64_t s = L - p;
+    bn_shr(m, n, X, nx, s);
+    rbit = bn_bit(X, nx, s - 1);
+    sticky = bn_any_below(X, nx, s - 1) || sticky_in;
+  } else {
+    bn_shl(m, n, X, nx, p - L);
+    rbit = 0;
+    sticky = sticky_in;
+  }
+  if (orc_round_decision(mode, sign, m[0] & 1, rbit, sticky)) {
+    uint64_t c = 1;
+    for (int i = 0; i < n; i++) { c += m[i]; m[i] = (uint32_t)c; c >>= 32; }
+    if (c) { /* carried out of the top: 2^p -> 2^(p-1), one binade up */
+      memset(m, 0, 4 * n);
+      m[n - 1] = 0x80000000u;
+      e++;
+    }
+  }
+  return e;
+}
+
+/* Map (sign, e) to an exponent word with the op's range rule. */
+static uint32_t pack_exp(int64_t e, int sign, int64_t emin_E, int64_t emax_E) {
+  int64_t E = e + ORC_BIAS;
+  if (E < emin_E) return (uint32_t)(C_PTINY - sign);
+  if (E > emax_E) return (uint32_t)(C_PINF - sign);
+  return (uint32_t)(2 * E + sign);
+}
+
+static void set_special(uint32_t *r_exp, uint32_t *r_limbs, int n, uint32_t code) {
+  *r_exp = code;
+  memset(r_limbs, 0, 4 * n);
+}
+
+/* -------------------------------- ops --------------------------------- */
+
+void orc_neg(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a, int n) {
+  int act = neg_tab[classify(a_exp)];
+  if (act == A_NEG) {
+    memmove(r_limbs, a, 4 * n);
+    *r_exp = a_exp ^ 1u;
+  } else set_special(r_exp, r_limbs, n, act);
+}
+
+/* magnitude add/sub with signs already resolved: result sign 'sa' for |a| >= |b| logic below */
+static void addsub_values(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a,
+                          uint32_t b_exp, const uint32_t *b, int n, int mode) {
+  int64_t p = 32 * (int64_t)n;
+  int sa = a_exp & 1, sb = b_exp & 1;
+  int64_t Ea = a_exp >> 1, Eb = b_exp >> 1;
+  /* scale of the integer mantissa: value = M * 2^(E - BIAS - p) */
+  const uint32_t *X = a, *Y = b;
+  int64_t Ex = Ea, Ey = Eb;
+  int sx = sa, sy = sb;
+  if (Ea < Eb || (Ea == Eb && bn_cmp(a, b, n) < 0)) { X = b; Y = a; Ex = Eb; Ey = Ea; sx = sb; sy = sa; }
+  int64_t d = Ex - Ey;
+  int64_t dc = d > p + 64 ? p + 64 : d;       /* clamping keeps every rounding outcome */
+  int nw = n + (int)((dc + 31) / 32) + 2;
+  uint32_t *Xs = (uint32_t *)calloc(nw, 4), *Ys = (uint32_t *)calloc(nw, 4), *T = (uint32_t *)calloc(nw, 4);
+  bn_shl(Xs, nw, X, n, dc);
+  memcpy(Ys, Y, 4 * n);
+  int64_t scale = Ey + (d - dc) - ORC_BIAS - p;  /* = Ex - dc - BIAS - p */
+  int negative_result = sx;
+  if (sx == sy) bn_add(T, Xs, Ys, nw);
+  else {
+    bn_sub(T, Xs, Ys, nw);  /* |X| >= |Y| by construction */
+    if (bn_len(T, nw) == 0) { set_special(r_exp, r_limbs, n, C_ZERO); goto out; }
+  }
+  {
+    int64_t e = round_to(r_limbs, n, T, nw, scale, 0, negative_result, mode);
+    *r_exp = pack_exp(e, negative_result, 5, 0x7FFFFFFFll);
+    if (*r_exp <= C_SNAN) memset(r_limbs, 0, 4 * n);
+  }
+out:
+  free(Xs); free(Ys); free(T);
+}
+
+void orc_addsub(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a,
+                uint32_t b_exp, const uint32_t *b, int n, int mode, int is_sub) {
+  uint32_t ca = classify(a_exp), cb = classify(b_exp);
+  int act = is_sub ? sub_tab[ca * 9 + cb] : add_tab[ca * 9 + cb];
+  uint32_t *tmp = (uint32_t *)malloc(4 * n);
+  switch (act) {
+    case A_ADD: case A_SUB:
+      addsub_values(r_exp, tmp, a_exp, a, is_sub ? (b_exp ^ 1u) : b_exp, b, n, mode);
+      memcpy(r_limbs, tmp, 4 * n);
+      break;
+    case A_COPY_A: memmove(r_limbs, a, 4 * n); *r_exp = a_exp; break;
+    case A_COPY_B: memmove(r_limbs, b, 4 * n); *r_exp = b_exp; break;
+    case A_NEG_B: memmove(r_limbs, b, 4 * n); *r_exp = b_exp ^ 1u; break;
+    default: set_special(r_exp, r_limbs, n, act);
+  }
+  free(tmp);
+}
+
+void orc_mul(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a,
+             uint32_t b_exp, const uint32_t *b, int n, int mode) {
+  int act = mul_tab[classify(a_exp) * 9 + classify(b_exp)];
+  if (act != A_MUL) { set_special(r_exp, r_limbs, n, act); return; }
+  int64_t p = 32 * (int64_t)n;
+  int sign = (a_exp ^ b_exp) & 1;
+  uint32_t *P = (uint32_t *)malloc(8 * n);
+  bn_mul(P, a, n, b, n);
+  int64_t scale = ((int64_t)(a_exp >> 1) - ORC_BIAS - p) + ((int64_t)(b_exp >> 1) - ORC_BIAS - p);
+  int64_t e = round_to(r_limbs, n, P, 2 * n, scale, 0, sign, mode);
+  *r_exp = pack_exp(e, sign, 5, 0x7FFFFFFFll);
+  if (*r_exp <= C_SNAN) memset(r_limbs, 0, 4 * n);
+  free(P);
+}
+
+void orc_div(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a,
+             uint32_t b_exp, const uint32_t *b, int n, int mode) {
+  int act = div_tab[classify(a_exp) * 9 + classify(b_exp)];
+  if (act != A_DIV) { set_special(r_exp, r_limbs, n, act); return; }
+  int64_t p = 32 * (int64_t)n;
+  int sign = (a_exp ^ b_exp) & 1;
+  /* Q = floor(A * 2^(p+64) / B), sticky = remainder != 0 */
+  int nu = 2 * n + 2;
+  uint32_t *U = (uint32_t *)calloc(nu, 4), *Q = (uint32_t *)calloc(nu, 4), *R = (uint32_t *)calloc(n, 4);
+  memcpy(U + n + 2, a, 4 * n);
+  bn_divmod(Q, R, U, nu, b, bn_len(b, n));
+  int sticky = bn_len(R, n) != 0;
+  int64_t scale = ((int64_t)(a_exp >> 1) - ORC_BIAS - p) - ((int64_t)(b_exp >> 1) - ORC_BIAS - p) - (p + 64);
+  int64_t e = round_to(r_limbs, n, Q, nu, scale, sticky, sign, mode);
+  *r_exp = pack_exp(e, sign, 10, 0x7FFFFFFEll);
+  if (*r_exp <= C_SNAN) memset(r_limbs, 0, 4 * n);
+  free(U); free(Q); free(R);
+}
+
+void orc_sqrt(uint32_t *r_exp, uint32_t *r_limbs, uint32_t a_exp, const uint32_t *a, int n, int mode) {
+  int act = sqrt_tab[classify(a_exp)];
+  if (act != A_SQRT) { set_special(r_exp, r_limbs, n, act); return; }
+  int64_t p = 32 * (int64_t)n;
+  /* value = A * 2^(E - BIAS - p).  Make the scaled exponent even: sqrt(A * 2^(2k)) * 2^t. */
+  int64_t sc = (int64_t)(a_exp >> 1) - ORC_BIAS - p;   /* value = A * 2^sc */
+  int64_t extra = 2 * p + 64;                          /* root gets >= p+32 bits */
+  if ((sc - extra) & 1) extra++;                       /* sc - extra even */
+  int nx = (int)((p + extra) / 32) + 2;
+  uint32_t *X = (uint32_t *)calloc(nx, 4), *S = (uint32_t *)calloc(nx / 2 + 1, 4);
+  bn_shl(X, nx, a, n, extra);
+  int inexact = bn_isqrt(S, X, nx);
+  int64_t scale = (sc - extra) / 2;
+  int64_t e = round_to(r_limbs, n, S, nx / 2 + 1, scale, inexact, 0, mode);
+  /* no range check in the reference's sqrt: |e| shrinks */
+  *r_exp = (uint32_t)(2 * (e + ORC_BIAS));
+  free(X); free(S);
+}
+
+/* ------------------------- array entry points ------------------------- */
+/* op: 0 neg, 1 add, 2 sub, 3 mul, 4 div, 5 sqrt.  Natural layout, n = precision/32. */
+void orc_op_array(int op, uint32_t count, uint32_t precision, int mode,
+                  uint32_t *r_exps, uint32_t *r_limbs,
+                  const uint32_t *a_exps, const uint32_t *a_limbs,
+                  const uint32_t *b_exps, const uint32_t *b_limbs) {
+  int n = (int)(precision / 32);
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t *rl = r_limbs + (size_t)i * n;
+    const uint32_t *al = a_limbs + (size_t)i * n;
+    const uint32_t *bl = b_limbs ? b_limbs + (size_t)i * n : NULL;
+    switch (op) {
+      case 0: orc_neg(&r_exps[i], rl, a_exps[i], al, n); break;
+      case 1: orc_addsub(&r_exps[i], rl, a_exps[i], al, b_exps[i], bl, n, mode, 0); break;
+      case 2: orc_addsub(&r_exps[i], rl, a_exps[i], al, b_exps[i], bl, n, mode, 1); break;
+      case 3: orc_mul(&r_exps[i], rl, a_exps[i], al, b_exps[i], bl, n, mode); break;
+      case 4: orc_div(&r_exps[i], rl, a_exps[i], al, b_exps[i], bl, n, mode); break;
+      case 5: orc_sqrt(&r_exps[i], rl, a_exps[i], al, n, mode); break;
+    }
+  }
+}
+
+/* big-integer self-test helpers exported for brute-force checks in tests */
+void orc_bn_divmod(uint32_t *q, uint32_t *rem, const uint32_t *u, int nu, const uint32_t *v, int nv) {
+  bn_divmod(q, rem, u, nu, v, nv);
+}
+int orc_bn_isqrt(uint32_t *r, const uint32_t *x, int nx) { return bn_isqrt(r, x, nx); }
+void orc_bn_mul(uint32_t *r, const uint32_t *a, int na, const uint32_t *b, int nb) { bn_mul(r, a, na, b, nb); }

@@ -1,0 +1,204 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes bindings for the oracle libraries built by ``oracle/Makefile``:
+
+* ``liboracle.so`` -- the exact-integer restatement (fpa_oracle.c), the MPFR gold
+  and the MPFR-backed ``fpa.h`` CPU backend (fpa_mpfr.c);
+* ``_ref/libreftests.so`` -- the reference's own MPFR test harness (tests.c) compiled
+  from its sources, used to draw the reference's generator cases with their gold.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import
+this module.  Arrays are numpy uint32 in the natural layout (value i's limbs at
+``limbs[i*n:(i+1)*n]``, little endian), as the reference's host API uses
+(fpa_management.h:17-47).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libreftests.so")
+
+OPS = {"neg": 0, "add": 1, "sub": 2, "mul": 3, "div": 4, "sqrt": 5}
+MODES = {"rz": 0, "ru": 1, "rd": 2, "rne": 3, "rna": 4}
+BIAS = 0x3FFFFFF3
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile the oracle (and the reference harness where its sources exist)."""
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        _lib = ctypes.CDLL(ORACLE_SO)
+        _lib.orc_op_array.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                      _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]
+        _lib.mpfr_gold_array.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                         _u32p, _u32p, ctypes.POINTER(ctypes.c_uint8),
+                                         _u32p, _u32p, _u32p, _u32p, ctypes.c_int]
+        _lib.orc_action.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+        _lib.orc_round_decision.argtypes = [ctypes.c_int] * 5
+        _lib.fpa_cpu_mpfr_version.restype = ctypes.c_char_p
+        _lib.fpa_cpu_struct_size.restype = ctypes.c_size_t
+        _lib.fpa_cpu_op.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        _lib.fpa_cpu_init.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+        _lib.fpa_cpu_clear.argtypes = [ctypes.c_void_p]
+        _lib.fpa_cpu_copy_in.argtypes = [ctypes.c_void_p, _u32p, _u32p]
+        _lib.fpa_cpu_copy_out.argtypes = [ctypes.c_void_p, _u32p, _u32p, ctypes.c_int]
+        _lib.fpa_cpu_set_threads.argtypes = [ctypes.c_int]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_u32p)
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def op(name, precision, a_exps, a_limbs, b_exps=None, b_limbs=None, mode="rne"):
+    """Exact-integer oracle over arrays; returns (r_exps, r_limbs)."""
+    a_exps, a_limbs, b_exps, b_limbs = _c(a_exps), _c(a_limbs), _c(b_exps), _c(b_limbs)
+    count = len(a_exps)
+    n = precision // 32
+    r_exps = np.zeros(count, np.uint32)
+    r_limbs = np.zeros(count * n, np.uint32)
+    lib().orc_op_array(OPS[name], count, precision, MODES[mode], _p(r_exps), _p(r_limbs),
+                       _p(a_exps), _p(a_limbs), _p(b_exps), _p(b_limbs))
+    return r_exps, r_limbs
+
+
+def mpfr_gold(name, precision, a_exps, a_limbs, b_exps=None, b_limbs=None, mode="rne", threads=0):
+    """MPFR gold; returns (r_exps, r_limbs, ok) where ok marks elements MPFR could evaluate."""
+    a_exps, a_limbs, b_exps, b_limbs = _c(a_exps), _c(a_limbs), _c(b_exps), _c(b_limbs)
+    count = len(a_exps)
+    n = precision // 32
+    r_exps = np.zeros(count, np.uint32)
+    r_limbs = np.zeros(count * n, np.uint32)
+    ok = np.zeros(count, np.uint8)
+    lib().mpfr_gold_array(OPS[name], count, precision, MODES[mode], _p(r_exps), _p(r_limbs),
+                          ok.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                          _p(a_exps), _p(a_limbs), _p(b_exps), _p(b_limbs), threads)
+    return r_exps, r_limbs, ok.astype(bool)
+
+
+def action(op_name, a_exp, b_exp=0):
+    return lib().orc_action(OPS[op_name], a_exp, b_exp)
+
+
+def round_decision(mode, sign, lsb, round_bit, sticky):
+    return lib().orc_round_decision(MODES[mode], sign, lsb, round_bit, sticky)
+
+
+def results_equal(ra, la, rb, lb, precision):
+    """Per-element equality: exponent words equal, limbs equal when the word is a value."""
+    n = precision // 32
+    ra = np.asarray(ra, np.uint32)
+    rb = np.asarray(rb, np.uint32)
+    la = np.asarray(la, np.uint32).reshape(-1, n)
+    lb = np.asarray(lb, np.uint32).reshape(-1, n)
+    eq = ra == rb
+    val = ra > 8
+    eq &= ~val | np.all(la == lb, axis=1)
+    return eq
+
+
+class CpuArray:
+    """MPFR-backed fpa.h CPU backend array (restates the -DCPU struct, fpa.h:28-34)."""
+
+    def __init__(self, count, precision):
+        self.buf = ctypes.create_string_buffer(lib().fpa_cpu_struct_size())
+        self.count, self.precision = count, precision
+        lib().fpa_cpu_init(self.buf, count, precision)
+
+    def load(self, exps, limbs):
+        lib().fpa_cpu_copy_in(self.buf, _p(_c(exps)), _p(_c(limbs)))
+
+    def store(self, op_name):
+        n = self.precision // 32
+        e = np.zeros(self.count, np.uint32)
+        l = np.zeros(self.count * n, np.uint32)
+        lib().fpa_cpu_copy_out(self.buf, _p(e), _p(l), OPS[op_name])
+        return e, l
+
+    def close(self):
+        if self.buf is not None:
+            lib().fpa_cpu_clear(self.buf)
+            self.buf = None
+
+
+def cpu_op(name, r, a, b=None, mode="rne"):
+    return lib().fpa_cpu_op(OPS[name], r.buf, a.buf, None if b is None else b.buf, MODES[mode])
+
+
+def cpu_set_threads(t):
+    lib().fpa_cpu_set_threads(int(t))
+
+
+# ------------------------- reference harness (tests.c) -------------------------
+class _Tests(ctypes.Structure):
+    # tests.h:10-29
+    _fields_ = [("test_type", ctypes.c_int), ("count", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("set_source_precision", ctypes.c_uint32), ("rounding", ctypes.c_int),
+                ("current", ctypes.c_uint32),
+                ("gold_exps", _u32p), ("gold_limbs", _u32p), ("r_exps", _u32p), ("r_limbs", _u32p),
+                ("a_exps", _u32p), ("a_limbs", _u32p), ("b_exps", _u32p), ("b_limbs", _u32p),
+                ("values", ctypes.c_void_p)]
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(REF_SO)
+        for name in ("add", "sub", "mul", "div", "sqrt", "neg"):
+            f = getattr(_ref, "tests_generate_" + name)
+            f.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+            f.restype = ctypes.POINTER(_Tests)
+        _ref.tests_free.argtypes = [ctypes.POINTER(_Tests)]
+    return _ref
+
+
+def ref_generate(name, count, precision):
+    """Draw `count` cases from the reference's own generator (tests.c:727-925).
+
+    Returns dict with a_exps, a_limbs, (b_exps, b_limbs,) gold_exps, gold_limbs.
+    The generator's GMP random state persists inside the loaded library, so successive
+    calls continue the reference's deterministic sequence.
+    """
+    t = getattr(ref_lib(), "tests_generate_" + name)(count, precision)
+    if not t or t.contents.count != count or t.contents.precision != precision or t.contents.current != count:
+        raise RuntimeError("reference harness returned an unusable tests_t")
+    n = precision // 32
+    s = t.contents
+
+    def arr(ptr, m):
+        return np.ctypeslib.as_array(ptr, shape=(m,)).copy()
+
+    out = {"a_exps": arr(s.a_exps, count), "a_limbs": arr(s.a_limbs, count * n),
+           "gold_exps": arr(s.gold_exps, count), "gold_limbs": arr(s.gold_limbs, count * n)}
+    if bool(s.b_exps):
+        out["b_exps"] = arr(s.b_exps, count)
+        out["b_limbs"] = arr(s.b_limbs, count * n)
+    ref_lib().tests_free(t)
+    # tests_set_mpfr leaves the limbs of zeros unset (tests.c:142-145): clear them
+    for k in ("a", "b", "gold"):
+        if k + "_exps" in out:
+            z = out[k + "_exps"] == 2
+            out[k + "_limbs"].reshape(-1, n)[z] = 0
+    return out

@@ -1,0 +1,44 @@
+"""Writes the committed golden fixtures (run here, where the reference sources exist).
+
+* ref_<op>_<p>.npz  -- cases drawn by the reference's own generator
+  (tests.c:727-925, compiled from its sources into oracle/_ref/) with the reference
+  harness's MPFR gold (round to nearest even).
+* mpfr_<op>_<p>.npz -- config-2-style operands (zeros, near-cancellation pairs) from the
+  package generator at precisions outside the reference's range, with MPFR gold.
+Usage: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+OPS = ["add", "sub", "mul", "div", "sqrt"]
+
+
+def main():
+    O.build()
+    for op in OPS:
+        for p, cnt in ((1024, 64), (2048, 48)):
+            g = O.ref_generate(op, cnt, p)
+            np.savez_compressed(os.path.join(HERE, f"ref_{op}_{p}.npz"), precision=np.uint32(p), **g)
+    import cubigfloat_amd as F
+    for op in OPS:
+        for p, cnt in ((256, 64), (4096, 24), (16384, 6)):
+            ae, al, be, bl = F.generate(2, 1000 + p, op, p, cnt, binary=op != "sqrt")
+            ge, gl, ok = O.mpfr_gold(op, p, ae, al, be, bl)
+            assert ok.all()
+            d = dict(a_exps=ae, a_limbs=al, gold_exps=ge, gold_limbs=gl, precision=np.uint32(p))
+            if be is not None:
+                d.update(b_exps=be, b_limbs=bl)
+            np.savez_compressed(os.path.join(HERE, f"mpfr_{op}_{p}.npz"), **d)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

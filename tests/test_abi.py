@@ -1,0 +1,127 @@
+"""CPU tests of the C ABI: the library loads, exports every declared symbol, and its host
+logic (dispatch tables, argument validation, scalar helpers, workload generator) behaves."""
+import ctypes
+import math
+import os
+import random
+import re
+
+import numpy as np
+import pytest
+
+import cubigfloat_amd as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(fpa_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert "fpa_mul" in names and "fpa_kernel_launch" in names
+    missing = [n for n in names if not hasattr(F.lib, n)]
+    assert not missing, missing
+
+
+def test_reference_api_symbols_present():
+    # fpa.h:43-63 of the reference
+    for n in ["fpa_init", "fpa_clear", "fpa_copy_array_to_device", "fpa_copy_array_to_host", "fpa_set_ui",
+              "fpa_set_si", "fpa_set_float", "fpa_set_double", "fpa_set", "fpa_sgn", "fpa_cmp", "fpa_neg",
+              "fpa_add", "fpa_sub", "fpa_mul", "fpa_div", "fpa_sqrt"]:
+        assert hasattr(F.lib, n), n
+
+
+def test_struct_layout_matches_reference():
+    # struct fpa_s {uint32 count; uint32 precision; uint32 *gpu_exps; uint32 *gpu_limbs;}
+    assert ctypes.sizeof(F.FpaStruct) == 24
+    assert F.FpaStruct.gpu_exps.offset == 8 and F.FpaStruct.gpu_limbs.offset == 16
+    assert ctypes.sizeof(F.FpaArguments) == 72
+
+
+def test_precision_validation_without_device():
+    s = F.FpaStruct()
+    for bad in (0, 32, 1000, 1536, 9216, 32768):
+        assert F.lib.fpa_init(ctypes.byref(s), 4, bad) == F.INVALID_PRECISIONS
+    for p in F.PRECISIONS:
+        assert F.lib.fpa_precision_supported(p) == 1
+
+
+def test_variant_tables():
+    for op in F.OPS:
+        for p in F.PRECISIONS:
+            vs = F.variants(op, p)
+            assert vs, (op, p)
+            for k, l in vs:
+                assert k * l == p // 32 and 64 % l == 0
+            assert F.get_variant(op, p) == 0
+    assert F.set_variant("mul", 4096, 99) == -1
+    assert F.set_variant("mul", 4096, 1) == 0 and F.get_variant("mul", 4096) == 1
+    F.set_variant("mul", 4096, 0)
+
+
+def test_tuning_file(tmp_path):
+    f = tmp_path / "tune.txt"
+    k, l = F.variants("mul", 4096)[1]
+    f.write_text(f"# op precision K L\n3 4096 {k} {l}\n3 4096 3 3\n")
+    assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == 1
+    assert F.get_variant("mul", 4096) == 1
+    F.set_variant("mul", 4096, 0)
+    assert F.lib.fpa_kernel_load_tuning(b"/nonexistent/file") == -1
+
+
+def test_scalar_helpers_vs_python():
+    rnd = random.Random(1)
+    alph = [0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF, 0xFFFFFFFE]
+    pick = lambda: rnd.choice(alph) if rnd.random() < 0.5 else rnd.getrandbits(32)
+    for _ in range(20000):
+        d1, d0 = pick() | 0x80000000, pick()
+        D = (d1 << 32) | d0
+        v = F.lib.fpa_scalar_recip_3by2(d1, d0)
+        assert v == ((1 << 96) - 1) // D - (1 << 32)
+        u2 = rnd.randrange(0, d1 + 1)
+        u1, u0 = pick(), pick()
+        if (u2 << 32 | u1) >= D:
+            continue
+        U = (u2 << 64) | (u1 << 32) | u0
+        assert F.lib.fpa_scalar_div3by2(u2, u1, u0, d1, d0, v) == U // D
+        x = rnd.getrandbits(64)
+        assert F.lib.fpa_scalar_isqrt64(x) == math.isqrt(x)
+    for m in range(5):
+        for s in (0, 1):
+            for lsb in (0, 1):
+                for rw in (0, 1, 0x7FFFFFFF, 0x80000000, 0x80000001, 0xFFFFFFFF):
+                    for st in (0, 1):
+                        from oracle import oracle as O
+                        exp = O.round_decision(["rz", "ru", "rd", "rne", "rna"][m], s, lsb, rw >> 31,
+                                               int((rw & 0x7FFFFFFF) != 0 or st))
+                        assert F.lib.fpa_scalar_round_inc(m, s, lsb, rw, st) == exp
+
+
+def test_workload_generator_properties():
+    p = 4096
+    n = p // 32
+    ae, al, be, bl = F.generate(2, 2, "mul", p, 20000)
+    vals = ae > 8
+    assert (al.reshape(-1, n)[vals][:, -1] >= 0x80000000).all()
+    e = (ae[vals].astype(np.int64) >> 1) - F.BIAS
+    assert e.min() >= -1024 and e.max() <= 1025
+    zero_rate = (ae == 2).mean()
+    assert 0.005 < zero_rate < 0.015
+    # regenerating a sub-range reproduces the same elements
+    ae2, al2, _, _ = F.generate(2, 2, "mul", p, 100, first=500)
+    assert (ae2 == ae[500:600]).all() and (al2 == al[500 * n:600 * n]).all()
+    # near-cancellation pairs: |b| within one binade of |a| with opposite sign for add
+    ae, al, be, bl = F.generate(2, 4, "add", p, 20000)
+    both = (ae > 8) & (be > 8)
+    close = both & (np.abs((ae.astype(np.int64) >> 1) - (be.astype(np.int64) >> 1)) <= 1) & ((ae ^ be) & 1 == 1)
+    assert 0.03 < close.mean() < 0.08

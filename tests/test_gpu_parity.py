@@ -1,0 +1,210 @@
+"""Parity of the HIP path (through the fpa.h C ABI) with the oracle and the golden fixtures.
+
+Bar: bit-exact -- exponent words equal, and limbs equal wherever the word is a value.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cubigfloat_amd as F
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OPS = ["add", "sub", "mul", "div", "sqrt"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check(op, p, ae, al, be, bl, mode="rne", tag=""):
+    say(f"[gpu] {op} p={p} count={len(ae)} mode={mode} {tag}")
+    re, rl = F.run_op(op, p, ae, al, be, bl, mode=F.MODES[mode])
+    oe, ol = O.op(op, p, ae, al, be, bl, mode=mode)
+    eq = O.results_equal(re, rl, oe, ol, p)
+    if not eq.all():
+        i = int(np.where(~eq)[0][0])
+        n = p // 32
+        msg = (f"{op} p={p} mode={mode} {tag}: {int((~eq).sum())} mismatches, first i={i} "
+               f"a_exp={ae[i]:#x} b_exp={(be[i] if be is not None else 0):#x} gpu_exp={re[i]:#x} oracle_exp={oe[i]:#x}\n"
+               f" gpu   ={rl.reshape(-1, n)[i][::-1][:6]}\n oracle={ol.reshape(-1, n)[i][::-1][:6]}")
+        pytest.fail(msg)
+
+
+def counts_for(p):
+    return {256: 2048, 512: 2048, 1024: 1024, 2048: 512, 3072: 256, 4096: 256, 5120: 128, 6144: 96,
+            7168: 96, 8192: 64, 16384: 24}[p]
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("p", F.PRECISIONS)
+def test_all_variants_config2_inputs(op, p):
+    """Every compiled (K, L) shape on config-2-style operands (zeros, cancellation pairs)."""
+    cnt = counts_for(p)
+    ae, al, be, bl = F.generate(2, 7 + p, op, p, cnt, binary=op != "sqrt")
+    default = F.get_variant(op, p)
+    try:
+        for v, (k, l) in enumerate(F.variants(op, p)):
+            F.set_variant(op, p, v)
+            check(op, p, ae, al, be, bl, tag=f"variant K={k} L={l}")
+    finally:
+        F.set_variant(op, p, default)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(HERE, "golden", "*.npz"))),
+                         ids=lambda s: os.path.basename(s))
+def test_golden_fixtures(path):
+    """Committed vectors: the reference generator's cases with its MPFR gold, and MPFR gold
+    at precisions outside the reference's range."""
+    z = np.load(path)
+    op = os.path.basename(path).split("_")[1]
+    p = int(z["precision"])
+    be = z["b_exps"] if "b_exps" in z else None
+    bl = z["b_limbs"] if "b_limbs" in z else None
+    say(f"[gpu] fixture {os.path.basename(path)}")
+    re, rl = F.run_op(op, p, z["a_exps"], z["a_limbs"], be, bl)
+    eq = O.results_equal(re, rl, z["gold_exps"], z["gold_limbs"], p)
+    assert eq.all(), f"{int((~eq).sum())} mismatches in {path}"
+
+
+@pytest.mark.parametrize("mode", ["rz", "ru", "rd", "rne", "rna"])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("p", [256, 1024, 4096])
+def test_rounding_modes(op, p, mode):
+    ae, al, be, bl = F.generate(2, 99, op, p, 512 if p < 4096 else 128, binary=op != "sqrt")
+    check(op, p, ae, al, be, bl, mode=mode)
+
+
+def adversarial(op, p, cnt, seed):
+    """Limbs from {0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF} like the reference's
+    generator (tests.c:173-181), exponent gaps around 0, 1, 2, 32, p, p+1 bits."""
+    rng = np.random.default_rng(seed)
+    n = p // 32
+    alph = np.array([0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF], np.uint32)
+
+    def mant():
+        m = alph[rng.integers(0, 5, size=(cnt, n))]
+        m[:, -1] |= np.uint32(0x80000000)
+        return m.reshape(-1)
+
+    gaps = np.array([0, 1, 2, 3, 31, 32, 33, p - 1, p, p + 1, p + 2, p + 40, 5 * p])
+    ea = O.BIAS + rng.integers(-50, 50, size=cnt)
+    eb = ea - gaps[rng.integers(0, len(gaps), size=cnt)] * rng.choice([-1, 1], size=cnt)
+    sa = rng.integers(0, 2, size=cnt)
+    sb = rng.integers(0, 2, size=cnt)
+    if op == "sqrt":
+        sa[:] = 0
+    a_exps = (2 * ea + sa).astype(np.uint32)
+    b_exps = (2 * eb + sb).astype(np.uint32)
+    return a_exps, mant(), b_exps, mant()
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("p", [256, 1024, 4096, 16384])
+def test_adversarial_limbs(op, p):
+    ae, al, be, bl = adversarial(op, p, 256 if p < 16384 else 32, 3 + p)
+    for mode in ("rne", "rz", "ru"):
+        check(op, p, ae, al, None if op == "sqrt" else be, None if op == "sqrt" else bl, mode=mode, tag="adversarial")
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_special_classes(op):
+    """All 9x9 class combinations (codes 2..8 and +-values)."""
+    p = 1024
+    n = p // 32
+    codes = [2 * (O.BIAS + 3), 2 * (O.BIAS + 3) + 1, 2, 3, 4, 5, 6, 7, 8]
+    ae = np.repeat(np.array(codes, np.uint32), 9)
+    be = np.tile(np.array(codes, np.uint32), 9)
+    rng = np.random.default_rng(5)
+    al = rng.integers(0, 2**32, size=81 * n, dtype=np.uint64).astype(np.uint32)
+    bl = rng.integers(0, 2**32, size=81 * n, dtype=np.uint64).astype(np.uint32)
+    al.reshape(81, n)[:, -1] |= np.uint32(0x80000000)
+    bl.reshape(81, n)[:, -1] |= np.uint32(0x80000000)
+    check(op, p, ae, al, be, bl, tag="classes")
+
+
+def test_special_classes_sqrt_neg():
+    p = 512
+    n = p // 32
+    codes = [2 * (O.BIAS + 3), 2 * (O.BIAS + 3) + 1, 2, 3, 4, 5, 6, 7, 8]
+    ae = np.array(codes, np.uint32)
+    al = np.full(9 * n, 0x80000000, np.uint32)
+    check("sqrt", p, ae, al, None, None, tag="classes")
+    check("neg", p, ae, al, None, None, tag="classes")
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_exponent_range_edges(op):
+    """Results at the TINY/INF thresholds of each op (the reference's exponent checks)."""
+    p = 512
+    n = p // 32
+    rng = np.random.default_rng(11)
+    cnt = 600
+    lo, hi, B = 5, 0x7FFFFFFF, O.BIAS
+    d = rng.integers(-3, 4, cnt)
+    half = np.arange(cnt) < cnt // 2
+    if op in ("add", "sub"):
+        Ea = np.where(half, lo + rng.integers(0, 6, cnt), hi - rng.integers(0, 3, cnt))
+        Eb = Ea - rng.integers(0, 3, cnt)
+    elif op == "mul":
+        Ea = rng.integers(B - 2**29, B + 2**29, cnt)
+        Eb = np.where(half, B + lo - Ea + d, B + hi - Ea + d)
+    else:
+        Ea = rng.integers(B - 2**29, B + 2**29, cnt)
+        Eb = np.where(half, Ea + B - 10 + d, Ea + B - hi + d)
+    Eb = np.clip(Eb, lo, hi)
+    ae = (2 * Ea + rng.integers(0, 2, cnt)).astype(np.uint32)
+    be = (2 * Eb + rng.integers(0, 2, cnt)).astype(np.uint32)
+    al = rng.integers(0, 2**32, size=cnt * n, dtype=np.uint64).astype(np.uint32)
+    bl = rng.integers(0, 2**32, size=cnt * n, dtype=np.uint64).astype(np.uint32)
+    al.reshape(cnt, n)[:, -1] |= np.uint32(0x80000000)
+    bl.reshape(cnt, n)[:, -1] |= np.uint32(0x80000000)
+    for mode in ("rne", "ru", "rd"):
+        check(op, p, ae, al, be, bl, mode=mode, tag="range")
+
+
+def test_count_zero_and_one():
+    for op in OPS:
+        for cnt in (1, 3):
+            ae, al, be, bl = F.generate(2, 5, op, 4096, cnt, binary=op != "sqrt")
+            check(op, 4096, ae, al, be, bl, tag=f"count={cnt}")
+    A = F.FpaArray(0, 1024)
+    R = F.FpaArray(0, 1024)
+    assert F.mul(R, A, A) == F.SUCCESS
+    A.close()
+    R.close()
+
+
+def test_aliasing_r_is_a():
+    """r may alias a or b: each value's inputs are read before its output is written."""
+    p = 2048
+    ae, al, be, bl = F.generate(2, 21, "mul", p, 300)
+    for op in ("add", "mul", "div"):
+        A = F.FpaArray(300, p)
+        B = F.FpaArray(300, p)
+        A.upload(ae, al)
+        B.upload(be, bl)
+        assert F.apply(op, A, A, B) == F.SUCCESS
+        re, rl = A.download()
+        oe, ol = O.op(op, p, ae, al, be, bl)
+        assert O.results_equal(re, rl, oe, ol, p).all(), op
+        A.close()
+        B.close()
+
+
+def test_invalid_arguments():
+    A = F.FpaArray(10, 1024)
+    B = F.FpaArray(11, 1024)
+    C = F.FpaArray(10, 2048)
+    assert F.add(A, A, B) == F.INVALID_COUNTS
+    assert F.mul(A, A, C) == F.INVALID_PRECISIONS
+    for x in (A, B, C):
+        x.close()
+    s = F.FpaStruct()
+    assert F.lib.fpa_init(F.ctypes.byref(s), 10, 1000) == F.INVALID_PRECISIONS
