@@ -150,6 +150,12 @@ def main():
 
     import cubigfloat_amd as F
     hip = Hip()
+    F.lib.fpa_ubench_mad.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]
+    blocks, iters = 256 * 4 * 8, 4096
+    ms = ctypes.c_float()
+    assert F.lib.fpa_ubench_mad(blocks, iters, ctypes.byref(ms)) == 0
+    mad_measured_t = blocks * 256 * iters * 8 / (ms.value / 1e3) / 1e12
+    print(f"[bench] v_mad_u64_u32 probe: {mad_measured_t:.2f} T/s ({ms.value:.2f} ms)", file=sys.stderr, flush=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     p, count = 4096, args.count
     print(f"[bench] mul {p}-bit x{count}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
@@ -197,7 +203,8 @@ def main():
                    "kernel_shape": F.variants("mul", p)[F.get_variant("mul", p)]},
         "roofline": {"bound": "valu", "achieved": head["tproducts_per_s"], "peak": MAD_PEAK_T,
                      "unit": "T 32x32->64 products/s", "frac": head["tproducts_per_s"] / MAD_PEAK_T,
-                     "traffic": None,
+                     "traffic": None, "peak_measured_mad_probe": mad_measured_t,
+                     "frac_of_measured": head["tproducts_per_s"] / mad_measured_t,
                      "note": "INT32 VALU multiply roofline (no MFMA, no HBM bound at 4096 bits); "
                              "peak = v_mad_u64_u32 at quarter rate, 256 CU x 16 lanes/clk x 2.4 GHz"},
         "cpu_baseline": cpu,

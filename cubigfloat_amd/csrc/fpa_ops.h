@@ -513,16 +513,19 @@ __device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g
     lds_load(Tr, Tl + g.gl * K);
     msub_step<K, L>(g, R, rtop, d, Tr, dtop);
     for (int guard = 0; rtop < 0 && guard < 6; guard++) {
-      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (T + (d-1)*2^(32pos)) */
+      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
       uint32_t c = g.add(R, R, Tr);
       rtop += (int64_t)c + dtop;
       d -= 1u;
-      wsync();
-      if (g.gl == 0) Tl[pos] = d;
-      wsync();
-      lds_load(Tr, Tl + g.gl * K);
-      c = g.add(R, R, Tr);
-      rtop += (int64_t)c + dtop;
+      uint32_t Y[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        bool here = g.gl * K + i == pos;
+        Y[i] = here ? d : 0u;
+        Tr[i] = here ? d : Tr[i];
+      }
+      c = g.add(R, R, Y);
+      rtop += (int64_t)c;
     }
     wsync();
     /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */

@@ -70,6 +70,11 @@ void *fpa_get_stream(void);
 /* 1 if precision (bits) is supported */
 int fpa_precision_supported(uint32_t precision);
 
+/* Diagnostics: throughput probe of the 32x32->64 multiply-add (v_mad_u64_u32) the
+ * mul/div/sqrt kernels use.  blocks x 256 lanes each run iters*8 multiply-adds; the
+ * kernel time in ms is stored in *ms.  Returns 0 on success. */
+int fpa_ubench_mad(uint32_t blocks, uint32_t iters, float *ms);
+
 /* Scalar helpers exported for host-side tests (same code as the kernels use). */
 uint32_t fpa_scalar_round_inc(int mode, uint32_t sign, uint32_t lsb, uint32_t round_word, uint32_t sticky);
 uint32_t fpa_scalar_recip_3by2(uint32_t d1, uint32_t d0);
