@@ -1,0 +1,50 @@
+/*
+ * Drop-in check in C: the reference's own test harness (tests.c: generators, MPFR gold,
+ * tests_compare_results) drives this library through the unmodified fpa.h API, the way
+ * the reference's fpa_test.c test_<op> functions do (fpa_test.c:207-327).
+ * tests_compare_results exits(1) on the first mismatch, so the exit status is the verdict.
+ *
+ * usage: drop_in <op> <count> <precision>     op = add|sub|mul|div|sqrt
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#include "fpa.h"
+#include "tests.h"
+
+int main(int argc, char **argv) {
+  if (argc != 4) { fprintf(stderr, "usage: %s op count precision\n", argv[0]); return 2; }
+  const char *op = argv[1];
+  uint32_t count = (uint32_t)atoi(argv[2]), precision = (uint32_t)atoi(argv[3]);
+  tests_t *t;
+  if (!strcmp(op, "add")) t = tests_generate_add(count, precision);
+  else if (!strcmp(op, "sub")) t = tests_generate_sub(count, precision);
+  else if (!strcmp(op, "mul")) t = tests_generate_mul(count, precision);
+  else if (!strcmp(op, "div")) t = tests_generate_div(count, precision);
+  else if (!strcmp(op, "sqrt")) t = tests_generate_sqrt(count, precision);
+  else { fprintf(stderr, "unknown op %s\n", op); return 2; }
+  if (!t) return 3;
+  fpa_t r, a, b;
+  if (fpa_init(r, count, precision) != fpa_success || fpa_init(a, count, precision) != fpa_success) return 4;
+  fpa_copy_array_to_device(a, t->a_exps, t->a_limbs);
+  fpa_result rc;
+  if (!strcmp(op, "sqrt")) rc = fpa_sqrt(r, a, fpa_round_ties_to_even);
+  else {
+    fpa_init(b, count, precision);
+    fpa_copy_array_to_device(b, t->b_exps, t->b_limbs);
+    if (!strcmp(op, "add")) rc = fpa_add(r, a, b, fpa_round_ties_to_even);
+    else if (!strcmp(op, "sub")) rc = fpa_sub(r, a, b, fpa_round_ties_to_even);
+    else if (!strcmp(op, "mul")) rc = fpa_mul(r, a, b, fpa_round_ties_to_even);
+    else rc = fpa_div(r, a, b, fpa_round_ties_to_even);
+  }
+  if (rc != fpa_success) { fprintf(stderr, "fpa_%s returned %d\n", op, rc); return 5; }
+  fpa_copy_array_to_host(r, t->r_exps, t->r_limbs);
+  tests_compare_results(t);
+  printf("%s tests passed: %u values at %u bits\n", op, count, precision);
+  tests_free(t);
+  fpa_clear(r);
+  fpa_clear(a);
+  if (strcmp(op, "sqrt")) fpa_clear(b);
+  return 0;
+}
