@@ -13,9 +13,14 @@
 namespace fpa {
 
 /* LDS words of scratch per value for each op (neg, add, sub, mul, div, sqrt) */
-template <int OP, int n>
+template <int OP, int K, int L>
 struct ScratchWords {
-  static constexpr int raw = OP == 0 ? 0 : (OP <= 2 ? n + 1 : (OP == 3 ? n : (OP == 4 ? n + 1 : 2 * n + 1)));
+  static constexpr int n = K * L;
+  static constexpr int raw = OP == 0 ? 0
+                           : OP <= 2 ? n + 1
+                           : OP == 3 ? 2 * L * mul_digits_per_lane(K)
+                           : OP == 4 ? n + 1
+                           : 2 * n + 1;
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
   static constexpr int value = ((raw + 3) / 4) * 4 + 4;
 };
@@ -23,7 +28,7 @@ struct ScratchWords {
 template <int K, int L, int OP>
 __global__ __launch_bounds__(256) void fpa_kernel(fpa_arguments a) {
   constexpr int n = K * L, VPW = 64 / L;
-  constexpr int SW = ScratchWords<OP, n>::value;
+  constexpr int SW = ScratchWords<OP, K, L>::value;
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
   Group<L> g;
   const int wib = threadIdx.x >> 6;

@@ -6,10 +6,11 @@
  *   add/sub  fpa_warp_add_sub.h:41-380 -> op_addsub: align by an LDS funnel shift with a
  *            round word + sticky below the mantissa, group add/sub with ballot carry
  *            resolution, renormalise (1 bit, or clz + LDS shift on cancellation), round.
- *   mul      fpa_warp_mul.h:14-139 -> op_mul: product-scanning by rounds; in round r
- *            every lane multiplies the broadcast limb chunk r of a (from LDS) by its own
- *            chunk of b into a 2K-word window; the low window of group lane 0 is final
- *            and feeds round/sticky; windows then shift down one lane (bpermute).
+ *   mul      fpa_warp_mul.h:14-139 -> op_mul: operands recoded to radix 2^27 so every
+ *            digit product accumulates into a 64-bit column sum with one v_mad_u64_u32;
+ *            rounds broadcast a's digit chunk r, windows of column sums shift down one
+ *            lane per round (group lane 0's low window is complete), carries are
+ *            normalised once at the end.
  *   div      fpa_warp_div.h:127-201 -> op_div: long division with 32-bit quotient
  *            digits; digit estimate by the 3-by-2 division of the top three remainder
  *            words by the top two divisor words (exact up to +2), group
@@ -267,79 +268,178 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
 }
 
 /* -------------------------------- mul -------------------------------- */
-/* lds: n words (the broadcast operand) */
+/*
+ * The product is formed in radix 2^27: each lane recodes its 32K mantissa bits into
+ * KD = ceil(32K/27) digits, so a digit product is < 2^54 and a 64-bit column
+ * accumulator takes every product of a column (at most 608 at 16384 bits) without
+ * overflow.  The inner loop is then one v_mad_u64_u32 per digit product, with carries
+ * propagated once at the end.
+ *
+ * Rounds r = 0..L-1: every lane multiplies digit chunk r of a (broadcast from group lane
+ * r) by its own chunk of b into a window of 2*KD column sums; group lane 0's low KD
+ * columns are complete and go to LDS; windows then shift down one lane.  After the last
+ * round each lane holds KD high columns.  Columns are normalised to digits (local carry
+ * chains, then the multi-bit carries between chunks, then a 1-bit ballot resolution),
+ * stored to LDS, and the mantissa, round word and sticky are read back as 32-bit words.
+ */
+constexpr int MUL_RADIX = 27;
+
+__host__ __device__ constexpr int mul_digits_per_lane(int K) { return (32 * K + MUL_RADIX - 1) / MUL_RADIX; }
+
+/* 32 bits at bit offset o of a digit string (radix 2^27) held in LDS, nd digits */
+__device__ __forceinline__ uint32_t digits_bits(const uint32_t *d, int nd, int o) {
+  int k = o / MUL_RADIX, off = o - k * MUL_RADIX;
+  uint64_t t = 0;
+  if (k < nd) t = d[k] >> off;
+  if (k + 1 < nd) t |= (uint64_t)d[k + 1] << (MUL_RADIX - off);
+  if (k + 2 < nd) t |= (uint64_t)d[k + 2] << (2 * MUL_RADIX - off);
+  return (uint32_t)t;
+}
+
+/* local radix-2^27 normalisation of KD column sums with carry-in c; returns the carry out */
+template <int KD>
+__device__ __forceinline__ uint64_t normalize_columns(const uint64_t (&col)[KD], uint32_t (&dg)[KD], uint64_t c) {
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    uint64_t v = col[k] + c;
+    dg[k] = (uint32_t)v & M;
+    c = v >> MUL_RADIX;
+  }
+  return c;
+}
+
+/* add a multi-bit carry c at digit 0 of a normalised chunk; returns the 1-bit overflow */
+template <int KD>
+__device__ __forceinline__ uint32_t add_carry_digits(uint32_t (&dg)[KD], uint64_t c) {
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    uint64_t v = (uint64_t)dg[k] + c;
+    dg[k] = (uint32_t)v & M;
+    c = v >> MUL_RADIX;
+  }
+  return (uint32_t)c;
+}
+
+/* lds: 2 * L * KD words (64-bit low columns, then digits) */
 template <int K, int L>
 __device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   constexpr int n = K * L;
+  constexpr int KD = mul_digits_per_lane(K);
+  constexpr int ND = KD * L;               /* digits per operand */
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+  static_assert(2 * L <= 64, "low and high chunk carries share one 64-bit mask");
+  static_assert(ND < 4096, "column sums of 27-bit digits must stay below 2^64");
   const int mode = (int)a.mode;
   uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
   uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
   if (act != FPA_A_MUL) { store_special<K>(a, s, g.gl, act); return; }
   const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
-  uint32_t b[K], t[K];
-  load_chunk(b, a.b_limbs + off);
-  load_chunk(t, a.a_limbs + off);
-  lds_store(lds + g.gl * K, t);
+
+  /* recode both operands into radix-2^27 digit chunks via LDS */
+  uint32_t A[KD], B[KD];
+  {
+    uint32_t t[K];
+    load_chunk(t, a.a_limbs + off);
+    lds_store(lds + g.gl * K, t);
+    wsync();
+#pragma unroll
+    for (int k = 0; k < KD; k++) A[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
+    wsync();
+    load_chunk(t, a.b_limbs + off);
+    lds_store(lds + g.gl * K, t);
+    wsync();
+#pragma unroll
+    for (int k = 0; k < KD; k++) B[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
+    wsync();
+  }
+
+  uint64_t W[2 * KD];
+#pragma unroll
+  for (int i = 0; i < 2 * KD; i++) W[i] = 0;
+  uint64_t *low = reinterpret_cast<uint64_t *>(lds);   /* ND 64-bit low column sums */
+  for (int r = 0; r < L; r++) {
+#pragma unroll
+    for (int i = 0; i < KD; i++) {
+      const uint32_t ai = g.bcast(A[i], r);
+#pragma unroll
+      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ai * B[j];
+    }
+    /* group lane 0's low window: columns r*KD .. r*KD+KD-1 are complete */
+    if (g.gl == 0) {
+#pragma unroll
+      for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
+    }
+    /* next window = own high half + next lane's low half */
+#pragma unroll
+    for (int i = 0; i < KD; i++) {
+      uint32_t lo = g.from_next((uint32_t)W[i]);
+      uint32_t hi = g.from_next((uint32_t)(W[i] >> 32));
+      W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
+      W[KD + i] = 0;
+    }
+  }
   wsync();
 
-  uint32_t W[2 * K];
+  /* normalise: chain of 2L chunks, low chunks 0..L-1 then high chunks 0..L-1 */
+  uint32_t dl[KD], dh[KD];
+  {
+    uint64_t cl[KD], ch[KD];
 #pragma unroll
-  for (int i = 0; i < 2 * K; i++) W[i] = 0;
-  uint32_t cc = 0, sticky = 0, rw = 0;
-  for (int r = 0; r < L; r++) {
-    uint32_t ar[K];
-    lds_load(ar, lds + r * K);
-    /* W += ar * b, row by row; row i's last carry lands in a word no earlier row wrote */
+    for (int i = 0; i < KD; i++) { cl[i] = low[g.gl * KD + i]; ch[i] = W[i]; }
+    wsync();   /* every lane has read its low columns before the digits overwrite them */
+    uint64_t co_l = normalize_columns<KD>(cl, dl, 0);
+    uint64_t co_h = normalize_columns<KD>(ch, dh, 0);
+    /* multi-bit carries from the preceding chunk */
+    uint32_t pl_lo = g.from_prev((uint32_t)co_l), pl_hi = g.from_prev((uint32_t)(co_l >> 32));
+    uint32_t ph_lo = g.from_prev((uint32_t)co_h), ph_hi = g.from_prev((uint32_t)(co_h >> 32));
+    uint64_t top_l = ((uint64_t)g.bcast((uint32_t)(co_l >> 32), L - 1) << 32) | g.bcast((uint32_t)co_l, L - 1);
+    uint64_t in_l = ((uint64_t)pl_hi << 32) | pl_lo;
+    uint64_t in_h = g.gl == 0 ? top_l : (((uint64_t)ph_hi << 32) | ph_lo);
+    uint32_t ol = add_carry_digits<KD>(dl, in_l);
+    uint32_t oh = add_carry_digits<KD>(dh, in_h);
+    /* 1-bit carries along the chain: bit t = low chunk t, bit L+t = high chunk t */
+    bool pl = true, ph = true;
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-      uint32_t c = 0;
-#pragma unroll
-      for (int j = 0; j < K; j++) {
-        uint64_t tt = (uint64_t)ar[i] * b[j] + W[i + j] + c;
-        W[i + j] = (uint32_t)tt;
-        c = (uint32_t)(tt >> 32);
-      }
-      W[i + K] = c;
-    }
-    /* group lane 0's low window holds final product limbs r*K .. r*K+K-1 */
-    if (r < L - 1) {
-#pragma unroll
-      for (int i = 0; i < K; i++) sticky |= W[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < K - 1; i++) sticky |= W[i];
-      rw = W[K - 1];
-    }
-    /* next window = own high half + next lane's low half + pending carry */
-    uint32_t c = cc;
-#pragma unroll
-    for (int i = 0; i < K; i++) {
-      uint32_t nx = g.from_next(W[i]);
-      uint64_t tt = (uint64_t)W[K + i] + nx + c;
-      W[i] = (uint32_t)tt;
-      c = (uint32_t)(tt >> 32);
-    }
-    cc = c;
+    for (int i = 0; i < KD; i++) { pl &= dl[i] == M; ph &= dh[i] == M; }
+    uint64_t G = g.ballot(ol != 0) | (g.ballot(oh != 0) << L);
+    uint64_t P = g.ballot(pl) | (g.ballot(ph) << L);
+    uint64_t X = G | P;
+    uint64_t S = X + G;                      /* 2L <= 64 bits; the value's top carry is zero */
+    uint64_t C = S ^ X ^ G;
+    add_carry_digits<KD>(dl, (C >> g.gl) & 1u);
+    add_carry_digits<KD>(dh, (C >> (L + g.gl)) & 1u);
   }
-  /* high half: window + carries from the lane below */
+  /* digits to LDS: [0, ND) low, [ND, 2ND) high */
+#pragma unroll
+  for (int i = 0; i < KD; i++) { lds[g.gl * KD + i] = dl[i]; lds[ND + g.gl * KD + i] = dh[i]; }
+  wsync();
+
+  /* P = A*B < 2^(2p); mantissa = top p bits from bit `base`, round word below, sticky */
+  constexpr int p = 32 * n;
+  const int topdig = (2 * p - 1) / MUL_RADIX, topoff = (2 * p - 1) % MUL_RADIX;
+  const bool full = (lds[topdig] >> topoff) & 1u;
+  const int base = full ? p : p - 1;
   uint32_t H[K];
 #pragma unroll
-  for (int i = 0; i < K; i++) H[i] = W[i];
-  g.add_lane_words(H, g.from_prev(cc));
-  sticky = g.bcast(sticky, 0);
-  rw = g.bcast(rw, 0);
+  for (int i = 0; i < K; i++) H[i] = digits_bits(lds, 2 * ND, base + 32 * (g.gl * K + i));
+  uint32_t rw = digits_bits(lds, 2 * ND, base - 32);
+  uint32_t loc = 0;
+  {
+    const int lim = base - 32;               /* sticky covers bits [0, lim) */
+#pragma unroll
+    for (int i = 0; i < KD; i++) {
+      int lo_bit = MUL_RADIX * (g.gl * KD + i);
+      if (lo_bit + MUL_RADIX <= lim) loc |= dl[i];
+      else if (lo_bit < lim) loc |= dl[i] & ((1u << (lim - lo_bit)) - 1u);
+    }
+  }
+  const uint32_t sticky = g.any(loc != 0) ? 1u : 0u;
+  wsync();
 
   const uint32_t sign = (ea ^ eb) & 1u;
-  int64_t E = (int64_t)(ea >> 1) + (int64_t)(eb >> 1) - FPA_BIAS_LL;
-  uint32_t top = g.bcast(H[K - 1], L - 1);
-  if (!(top >> 31)) {
-    uint32_t inb = g.from_prev(H[K - 1] >> 31, rw >> 31);
-#pragma unroll
-    for (int i = K - 1; i > 0; i--) H[i] = (H[i] << 1) | (H[i - 1] >> 31);
-    H[0] = (H[0] << 1) | inb;
-    rw <<= 1;
-    E -= 1;
-  }
+  int64_t E = (int64_t)(ea >> 1) + (int64_t)(eb >> 1) - FPA_BIAS_LL - (full ? 0 : 1);
   if (round_mantissa<K, L>(g, H, mode, sign, rw, sticky)) E += 1;
   store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), H);
 }
