@@ -210,3 +210,113 @@ def run_op(op, precision, a_exps, a_limbs, b_exps=None, b_limbs=None, mode=ROUND
         for x in (A, B, R):
             if x is not None:
                 x.close()
+
+
+# ------------------------------ batches (fpa_batch.h) ------------------------------
+_vp = ctypes.c_void_p
+lib.fpa_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8), _u32p,
+                                 ctypes.POINTER(ctypes.c_uint8)]
+lib.fpa_batch_set_operands.argtypes = [_vp, _u32p, _u32p, _u32p, _u32p]
+lib.fpa_batch_run.argtypes = [_vp, _vp]
+lib.fpa_batch_run_plain.argtypes = [_vp, _vp]
+lib.fpa_batch_get_results.argtypes = [_vp, _vp, _u32p, _u32p]
+lib.fpa_batch_destroy.argtypes = [_vp]
+for _n in ("fpa_batch_segments", "fpa_batch_tiles", "fpa_batch_grid"):
+    getattr(lib, _n).argtypes = [_vp]
+    getattr(lib, _n).restype = ctypes.c_uint32
+lib.fpa_batch_device_bytes.argtypes = [_vp]
+lib.fpa_batch_device_bytes.restype = ctypes.c_size_t
+lib.fpa_batch_total_limbs.argtypes = [_vp]
+lib.fpa_batch_total_limbs.restype = ctypes.c_uint64
+lib.fpa_batch_precision_supported.argtypes = [ctypes.c_uint32]
+lib.fpa_dot_chains.argtypes = [_P, _P, _P, _u32p, ctypes.c_uint32, ctypes.c_int]
+lib.fpa_dot_plan_create.argtypes = [ctypes.POINTER(_vp), _u32p, ctypes.c_uint32]
+lib.fpa_dot_plan_run.argtypes = [_vp, _P, _P, _P, ctypes.c_int, _vp]
+lib.fpa_dot_plan_destroy.argtypes = [_vp]
+
+BATCH_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192, 16384)
+
+
+class Batch:
+    """A planned mixed-precision batch (fpa_batch_t): one captured persistent launch."""
+
+    def __init__(self, ops, precisions, modes=None):
+        ops = np.ascontiguousarray([OPS[o] if isinstance(o, str) else o for o in ops], dtype=np.uint8)
+        precisions = np.ascontiguousarray(precisions, dtype=np.uint32)
+        modes = np.ascontiguousarray(np.full(len(ops), ROUND_TIES_TO_EVEN) if modes is None else modes, dtype=np.uint8)
+        self.h = _vp()
+        rc = lib.fpa_batch_create(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  _ptr(precisions), modes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        if rc != SUCCESS:
+            raise ValueError(f"fpa_batch_create returned {rc}")
+        self.ops, self.precisions, self.nops = ops, precisions, len(ops)
+        self.total_limbs = lib.fpa_batch_total_limbs(self.h)
+
+    def set_operands(self, a_exps, a_limbs, b_exps=None, b_limbs=None):
+        c = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.uint32)
+        a_exps, a_limbs, b_exps, b_limbs = c(a_exps), c(a_limbs), c(b_exps), c(b_limbs)
+        rc = lib.fpa_batch_set_operands(self.h, _ptr(a_exps), _ptr(a_limbs), _ptr(b_exps), _ptr(b_limbs))
+        if rc != SUCCESS:
+            raise ValueError(f"fpa_batch_set_operands returned {rc}")
+
+    def run(self, stream=None):
+        return lib.fpa_batch_run(self.h, stream)
+
+    def run_plain(self, stream=None):
+        return lib.fpa_batch_run_plain(self.h, stream)
+
+    def results(self, stream=None):
+        e = np.empty(self.nops, np.uint32)
+        l = np.empty(self.total_limbs, np.uint32)
+        lib.fpa_batch_get_results(self.h, stream, _ptr(e), _ptr(l))
+        return e, l
+
+    def info(self):
+        return {"segments": lib.fpa_batch_segments(self.h), "tiles": lib.fpa_batch_tiles(self.h),
+                "grid": lib.fpa_batch_grid(self.h), "device_bytes": lib.fpa_batch_device_bytes(self.h)}
+
+    def close(self):
+        if self.h:
+            lib.fpa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DotPlan:
+    """fpa_dot_plan_t: chain offsets and longest-first order resident in device memory."""
+
+    def __init__(self, lengths):
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+        self.h = _vp()
+        assert lib.fpa_dot_plan_create(ctypes.byref(self.h), _ptr(self.lengths), len(self.lengths)) == SUCCESS
+
+    def run(self, r, x, y, mode=ROUND_TIES_TO_EVEN, stream=None):
+        return lib.fpa_dot_plan_run(self.h, r.ref, x.ref, y.ref, mode, stream)
+
+    def close(self):
+        if self.h:
+            lib.fpa_dot_plan_destroy(self.h)
+            self.h = None
+
+
+def dot_chains(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode=ROUND_TIES_TO_EVEN):
+    """Upload, run fpa_dot_chains, download. Returns (r_exps, r_limbs)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    total = int(lengths.sum())
+    X, Y, R = FpaArray(total, precision), FpaArray(total, precision), FpaArray(len(lengths), precision)
+    try:
+        if total:
+            X.upload(x_exps, x_limbs)
+            Y.upload(y_exps, y_limbs)
+        rc = lib.fpa_dot_chains(R.ref, X.ref, Y.ref, _ptr(lengths), len(lengths), mode)
+        if rc != SUCCESS:
+            raise RuntimeError(f"fpa_dot_chains returned {rc}")
+        return R.download()
+    finally:
+        for a in (X, Y, R):
+            a.close()

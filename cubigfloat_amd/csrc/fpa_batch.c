@@ -1,0 +1,354 @@
+/*
+ * Host side of fpa_batch.h: batch planning (segments, operand pools, cost-sorted tiles,
+ * hipGraph capture of the persistent kernel) and dot-chain plans.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/fpa_batch.h"
+#include "fpa_scalar.h"
+
+#define FPA_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "\nCall \"%s\" failed from %s, line %d: %s\n", #call, __FILE__,     \
+              __LINE__, hipGetErrorString(e_));                                            \
+      exit(1);                                                                             \
+    }                                                                                      \
+  } while (0)
+
+int fpa_impl_batch_kind(int op, uint32_t precision);
+int fpa_impl_batch_kind_shape(int kind, int *op, uint32_t *precision, int *K, int *L);
+int fpa_impl_batch_blocks_per_cu(void);
+int fpa_impl_batch_launch(const fpa_batch_segment *segs, const void *tiles, uint32_t ntiles, uint32_t grid,
+                          void *stream);
+int fpa_impl_dot_launch(const fpa_dot_arguments *d, void *stream);
+void *fpa_get_stream(void);
+
+typedef struct {
+  int op, mode, kind, K, L;
+  uint32_t precision, count;
+  size_t off[6];           /* pool offsets: r_exps, r_limbs, a_exps, a_limbs, b_exps, b_limbs */
+} seg_host;
+
+typedef struct { uint32_t seg, first; float cost; } tile_host;
+
+struct fpa_batch_s {
+  uint32_t nops, nseg, ntiles, grid;
+  uint32_t *op_seg, *op_slot;
+  uint64_t *op_off;        /* packed host limb offset of each op */
+  uint64_t total_limbs;
+  seg_host *sh;
+  uint8_t *pool;           /* device */
+  size_t pool_bytes;
+  uint8_t *mirror;         /* pinned host copy of the pool layout */
+  fpa_batch_segment *segs_dev;
+  uint32_t *tiles_dev;     /* ntiles x (seg, first) */
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+int fpa_batch_precision_supported(uint32_t precision) { return fpa_impl_batch_kind(FPA_OP_MUL, precision) >= 0; }
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+/* relative time one wavefront spends on one tile of a kind (orders tiles, longest first) */
+static float tile_cost(int op, int K, int L) {
+  int n = K * L, KD = (32 * K + 26) / 27;
+  switch (op) {
+    case FPA_OP_MUL: return (float)L * KD * KD + 40.0f * KD;
+    case FPA_OP_DIV: return (float)(n + 1) * (3 * K + 40);
+    case FPA_OP_SQRT: return (float)n * (3 * K + 50);
+    default: return 8.0f * K + 60.0f;
+  }
+}
+
+static int tile_cmp(const void *x, const void *y) {
+  const tile_host *a = (const tile_host *)x, *b = (const tile_host *)y;
+  if (a->cost != b->cost) return a->cost > b->cost ? -1 : 1;
+  if (a->seg != b->seg) return a->seg < b->seg ? -1 : 1;
+  return a->first < b->first ? -1 : (a->first > b->first);
+}
+
+fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops, const uint32_t *precisions,
+                            const uint8_t *modes) {
+  *out = NULL;
+  for (uint32_t i = 0; i < nops; i++) {
+    if (ops[i] < FPA_OP_ADD || ops[i] > FPA_OP_SQRT || modes[i] > 4) return fpa_math_error;
+    if (fpa_impl_batch_kind(ops[i], precisions[i]) < 0) return fpa_invalid_precisions;
+  }
+  struct fpa_batch_s *b = (struct fpa_batch_s *)calloc(1, sizeof *b);
+  b->nops = nops;
+  b->op_seg = (uint32_t *)malloc(sizeof(uint32_t) * (nops + 1));
+  b->op_slot = (uint32_t *)malloc(sizeof(uint32_t) * (nops + 1));
+  b->op_off = (uint64_t *)malloc(sizeof(uint64_t) * (nops + 1));
+  int cap = 16;
+  b->sh = (seg_host *)calloc(cap, sizeof(seg_host));
+  /* segments keyed by (op, precision, mode), in first-appearance order */
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < nops; i++) {
+    uint32_t s;
+    for (s = 0; s < b->nseg; s++)
+      if (b->sh[s].op == ops[i] && b->sh[s].precision == precisions[i] && b->sh[s].mode == modes[i]) break;
+    if (s == b->nseg) {
+      if ((int)b->nseg == cap) { cap *= 2; b->sh = (seg_host *)realloc(b->sh, cap * sizeof(seg_host)); }
+      memset(&b->sh[s], 0, sizeof(seg_host));
+      b->sh[s].op = ops[i];
+      b->sh[s].precision = precisions[i];
+      b->sh[s].mode = modes[i];
+      b->sh[s].kind = fpa_impl_batch_kind(ops[i], precisions[i]);
+      int o;
+      uint32_t pp;
+      fpa_impl_batch_kind_shape(b->sh[s].kind, &o, &pp, &b->sh[s].K, &b->sh[s].L);
+      b->nseg++;
+    }
+    b->op_seg[i] = s;
+    b->op_slot[i] = b->sh[s].count++;
+    b->op_off[i] = off;
+    off += precisions[i] / 32;
+  }
+  b->op_off[nops] = off;
+  b->total_limbs = off;
+  /* pool layout */
+  size_t pos = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    seg_host *g = &b->sh[s];
+    size_t ne = (size_t)g->count * 4, nl = (size_t)g->count * (g->precision / 32) * 4;
+    int binary = g->op != FPA_OP_SQRT;
+    size_t sizes[6] = { ne, nl, ne, nl, binary ? ne : 0, binary ? nl : 0 };
+    for (int k = 0; k < 6; k++) { g->off[k] = pos; pos += align256(sizes[k] + 16); }
+  }
+  b->pool_bytes = pos ? pos : 256;
+  FPA_HIP(hipMalloc((void **)&b->pool, b->pool_bytes));
+  FPA_HIP(hipHostMalloc((void **)&b->mirror, b->pool_bytes, hipHostMallocDefault));
+  /* device segment descriptors */
+  fpa_batch_segment *sd = (fpa_batch_segment *)calloc(b->nseg ? b->nseg : 1, sizeof(fpa_batch_segment));
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    seg_host *g = &b->sh[s];
+    fpa_arguments *a = &sd[s].args;
+    a->count = g->count;
+    a->precision = g->precision;
+    a->mode = (fpa_rounding_mode)g->mode;
+    a->r_exps = (uint32_t *)(b->pool + g->off[0]);
+    a->r_limbs = (uint32_t *)(b->pool + g->off[1]);
+    a->a_exps = (uint32_t *)(b->pool + g->off[2]);
+    a->a_limbs = (uint32_t *)(b->pool + g->off[3]);
+    a->b_exps = (uint32_t *)(b->pool + (g->op == FPA_OP_SQRT ? g->off[2] : g->off[4]));
+    a->b_limbs = (uint32_t *)(b->pool + (g->op == FPA_OP_SQRT ? g->off[3] : g->off[5]));
+    sd[s].kind = (uint32_t)g->kind;
+  }
+  FPA_HIP(hipMalloc((void **)&b->segs_dev, sizeof(fpa_batch_segment) * (b->nseg ? b->nseg : 1)));
+  FPA_HIP(hipMemcpy(b->segs_dev, sd, sizeof(fpa_batch_segment) * b->nseg, hipMemcpyHostToDevice));
+  free(sd);
+  /* tiles: one wavefront's values of one segment, most expensive first */
+  uint64_t nt = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) nt += (b->sh[s].count + (64 / b->sh[s].L) - 1) / (64 / b->sh[s].L);
+  tile_host *th = (tile_host *)malloc(sizeof(tile_host) * (nt ? nt : 1));
+  uint64_t k = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    uint32_t vpw = 64 / b->sh[s].L;
+    float c = tile_cost(b->sh[s].op, b->sh[s].K, b->sh[s].L);
+    for (uint32_t f = 0; f < b->sh[s].count; f += vpw) { th[k].seg = s; th[k].first = f; th[k].cost = c; k++; }
+  }
+  qsort(th, nt, sizeof(tile_host), tile_cmp);
+  uint32_t *tt = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (nt ? nt : 1));
+  for (uint64_t i = 0; i < nt; i++) { tt[2 * i] = th[i].seg; tt[2 * i + 1] = th[i].first; }
+  b->ntiles = (uint32_t)nt;
+  FPA_HIP(hipMalloc((void **)&b->tiles_dev, sizeof(uint32_t) * 2 * (nt ? nt : 1)));
+  FPA_HIP(hipMemcpy(b->tiles_dev, tt, sizeof(uint32_t) * 2 * nt, hipMemcpyHostToDevice));
+  free(th);
+  free(tt);
+  /* persistent grid: every resident block slot of the device, no more than the tiles need */
+  int dev = 0, cus = 0;
+  FPA_HIP(hipGetDevice(&dev));
+  FPA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint64_t grid = (uint64_t)cus * fpa_impl_batch_blocks_per_cu();
+  uint64_t need = (nt + 3) / 4;
+  b->grid = (uint32_t)(grid < need ? grid : (need ? need : 1));
+  /* capture one launch in a graph */
+  hipStream_t cs;
+  FPA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  FPA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = fpa_impl_batch_launch(b->segs_dev, b->tiles_dev, b->ntiles, b->grid, cs);
+  FPA_HIP(hipStreamEndCapture(cs, &b->graph));
+  if (rc != 0) { fprintf(stderr, "batch kernel capture failed: %d\n", rc); exit(1); }
+  FPA_HIP(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+  FPA_HIP(hipStreamDestroy(cs));
+  *out = b;
+  return fpa_success;
+}
+
+static void scatter_operand(struct fpa_batch_s *b, int which, const uint32_t *exps, const uint32_t *limbs) {
+  for (uint32_t i = 0; i < b->nops; i++) {
+    seg_host *g = &b->sh[b->op_seg[i]];
+    if (which == 1 && g->op == FPA_OP_SQRT) continue;
+    uint32_t n = g->precision / 32, slot = b->op_slot[i];
+    uint32_t *e = (uint32_t *)(b->mirror + g->off[2 + 2 * which]);
+    uint32_t *l = (uint32_t *)(b->mirror + g->off[3 + 2 * which]);
+    e[slot] = exps[i];
+    memcpy(l + (size_t)slot * n, limbs + b->op_off[i], (size_t)n * 4);
+  }
+}
+
+fpa_result fpa_batch_set_operands(fpa_batch_t b, const uint32_t *a_exps, const uint32_t *a_limbs,
+                                  const uint32_t *b_exps, const uint32_t *b_limbs) {
+  scatter_operand(b, 0, a_exps, a_limbs);
+  int binary = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) binary |= b->sh[s].op != FPA_OP_SQRT;
+  if (binary) {
+    if (!b_exps || !b_limbs) return fpa_invalid_counts;
+    scatter_operand(b, 1, b_exps, b_limbs);
+  }
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    seg_host *g = &b->sh[s];
+    size_t ne = (size_t)g->count * 4, nl = ne * (g->precision / 32);
+    for (int k = 2; k < (g->op == FPA_OP_SQRT ? 4 : 6); k++) {
+      size_t bytes = (k & 1) ? nl : ne;
+      FPA_HIP(hipMemcpy(b->pool + g->off[k], b->mirror + g->off[k], bytes, hipMemcpyHostToDevice));
+    }
+  }
+  return fpa_success;
+}
+
+fpa_result fpa_batch_run(fpa_batch_t b, void *stream) {
+  FPA_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return fpa_success;
+}
+
+fpa_result fpa_batch_run_plain(fpa_batch_t b, void *stream) {
+  fpa_batch_segment *sd = (fpa_batch_segment *)malloc(sizeof(fpa_batch_segment) * (b->nseg ? b->nseg : 1));
+  FPA_HIP(hipMemcpy(sd, b->segs_dev, sizeof(fpa_batch_segment) * b->nseg, hipMemcpyDeviceToHost));
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    int op = b->sh[s].op;
+    int rc = fpa_kernel_launch(op, &sd[s].args, fpa_kernel_get_variant(op, b->sh[s].precision), stream);
+    if (rc != 0) { fprintf(stderr, "plain launch of segment %u failed: %d\n", s, rc); exit(1); }
+  }
+  free(sd);
+  return fpa_success;
+}
+
+fpa_result fpa_batch_get_results(fpa_batch_t b, void *stream, uint32_t *r_exps, uint32_t *r_limbs) {
+  FPA_HIP(hipStreamSynchronize((hipStream_t)stream));
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    seg_host *g = &b->sh[s];
+    size_t ne = (size_t)g->count * 4, nl = ne * (g->precision / 32);
+    FPA_HIP(hipMemcpy(b->mirror + g->off[0], b->pool + g->off[0], ne, hipMemcpyDeviceToHost));
+    FPA_HIP(hipMemcpy(b->mirror + g->off[1], b->pool + g->off[1], nl, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < b->nops; i++) {
+    seg_host *g = &b->sh[b->op_seg[i]];
+    uint32_t n = g->precision / 32, slot = b->op_slot[i];
+    r_exps[i] = ((uint32_t *)(b->mirror + g->off[0]))[slot];
+    memcpy(r_limbs + b->op_off[i], (uint32_t *)(b->mirror + g->off[1]) + (size_t)slot * n, (size_t)n * 4);
+  }
+  return fpa_success;
+}
+
+fpa_result fpa_batch_destroy(fpa_batch_t b) {
+  if (!b) return fpa_success;
+  FPA_HIP(hipGraphExecDestroy(b->exec));
+  FPA_HIP(hipGraphDestroy(b->graph));
+  FPA_HIP(hipFree(b->pool));
+  FPA_HIP(hipFree(b->segs_dev));
+  FPA_HIP(hipFree(b->tiles_dev));
+  FPA_HIP(hipHostFree(b->mirror));
+  free(b->op_seg);
+  free(b->op_slot);
+  free(b->op_off);
+  free(b->sh);
+  free(b);
+  return fpa_success;
+}
+
+uint32_t fpa_batch_segments(fpa_batch_t b) { return b->nseg; }
+uint32_t fpa_batch_tiles(fpa_batch_t b) { return b->ntiles; }
+uint32_t fpa_batch_grid(fpa_batch_t b) { return b->grid; }
+size_t fpa_batch_device_bytes(fpa_batch_t b) {
+  return b->pool_bytes + sizeof(fpa_batch_segment) * b->nseg + 8 * (size_t)b->ntiles;
+}
+uint64_t fpa_batch_total_limbs(fpa_batch_t b) { return b->total_limbs; }
+
+/* ------------------------------ dot chains ------------------------------ */
+struct fpa_dot_plan_s {
+  uint32_t count;
+  uint64_t total;
+  uint64_t *offsets_dev;
+  uint32_t *order_dev;
+};
+
+typedef struct { uint32_t len, idx; } len_idx;
+static int len_cmp(const void *x, const void *y) {
+  const len_idx *a = (const len_idx *)x, *b = (const len_idx *)y;
+  if (a->len != b->len) return a->len > b->len ? -1 : 1;
+  return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+
+fpa_result fpa_dot_plan_create(fpa_dot_plan_t *out, const uint32_t *lengths, uint32_t count) {
+  struct fpa_dot_plan_s *p = (struct fpa_dot_plan_s *)calloc(1, sizeof *p);
+  p->count = count;
+  uint64_t *off = (uint64_t *)malloc(sizeof(uint64_t) * (count + 1));
+  len_idx *li = (len_idx *)malloc(sizeof(len_idx) * (count ? count : 1));
+  uint32_t *ord = (uint32_t *)malloc(sizeof(uint32_t) * (count ? count : 1));
+  off[0] = 0;
+  for (uint32_t j = 0; j < count; j++) { off[j + 1] = off[j] + lengths[j]; li[j].len = lengths[j]; li[j].idx = j; }
+  qsort(li, count, sizeof(len_idx), len_cmp);
+  for (uint32_t j = 0; j < count; j++) ord[j] = li[j].idx;
+  p->total = off[count];
+  FPA_HIP(hipMalloc((void **)&p->offsets_dev, sizeof(uint64_t) * (count + 1)));
+  FPA_HIP(hipMalloc((void **)&p->order_dev, sizeof(uint32_t) * (count ? count : 1)));
+  FPA_HIP(hipMemcpy(p->offsets_dev, off, sizeof(uint64_t) * (count + 1), hipMemcpyHostToDevice));
+  FPA_HIP(hipMemcpy(p->order_dev, ord, sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+  free(off);
+  free(li);
+  free(ord);
+  *out = p;
+  return fpa_success;
+}
+
+fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, void *stream) {
+  if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
+  if (fpa_impl_batch_kind(FPA_OP_MUL, x->precision) < 0) return fpa_invalid_precisions;
+  if (x->count != y->count || (uint64_t)x->count != p->total || r->count != p->count) return fpa_invalid_counts;
+  fpa_dot_arguments d;
+  memset(&d, 0, sizeof d);
+  d.count = p->count;
+  d.precision = x->precision;
+  d.mode = (int)mode;
+  d.r_exps = r->gpu_exps;
+  d.r_limbs = r->gpu_limbs;
+  d.x_exps = x->gpu_exps;
+  d.x_limbs = x->gpu_limbs;
+  d.y_exps = y->gpu_exps;
+  d.y_limbs = y->gpu_limbs;
+  d.offsets = p->offsets_dev;
+  d.order = p->order_dev;
+  int rc = fpa_impl_dot_launch(&d, stream);
+  if (rc != 0) { fprintf(stderr, "dot kernel launch failed: %d\n", rc); exit(1); }
+  return fpa_success;
+}
+
+fpa_result fpa_dot_plan_destroy(fpa_dot_plan_t p) {
+  if (!p) return fpa_success;
+  FPA_HIP(hipFree(p->offsets_dev));
+  FPA_HIP(hipFree(p->order_dev));
+  free(p);
+  return fpa_success;
+}
+
+fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, uint32_t count, fpa_rounding_mode mode) {
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < count; j++) total += lengths[j];
+  if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
+  if (fpa_impl_batch_kind(FPA_OP_MUL, x->precision) < 0) return fpa_invalid_precisions;
+  if (x->count != y->count || (uint64_t)x->count != total || r->count != count) return fpa_invalid_counts;
+  fpa_dot_plan_t p;
+  fpa_dot_plan_create(&p, lengths, count);
+  void *st = fpa_get_stream();
+  fpa_result rc = fpa_dot_plan_run(p, r, x, y, mode, st);
+  FPA_HIP(hipStreamSynchronize((hipStream_t)st));
+  fpa_dot_plan_destroy(p);
+  return rc;
+}

@@ -27,7 +27,7 @@ struct ScratchWords {
 
 template <int K, int L, int OP>
 __global__ __launch_bounds__(256) void fpa_kernel(fpa_arguments a) {
-  constexpr int n = K * L, VPW = 64 / L;
+  constexpr int VPW = 64 / L;
   constexpr int SW = ScratchWords<OP, K, L>::value;
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
   Group<L> g;

@@ -130,41 +130,45 @@ __device__ __forceinline__ void store_special(const fpa_arguments &a, const Slot
   store_result(a, s, gl, code, z);
 }
 
+/* Loads this lane's chunk of value s.v of an operand array. */
+template <int K, int L>
+__device__ __forceinline__ void load_value(const Group<L> &g, const uint32_t *limbs, const Slot &s, uint32_t (&x)[K]) {
+  load_chunk(x, limbs + (size_t)s.v * (K * L) + (size_t)g.gl * K);
+}
+
 /* ------------------------------- neg ------------------------------- */
 template <int K, int L>
 __device__ void op_neg(const fpa_arguments &a, const Slot &s, const Group<L> &g) {
-  constexpr int n = K * L;
   uint32_t ea = a.a_exps[s.v];
   uint32_t act = k_neg_tab[fpa_classify(ea)];
   uint32_t x[K];
-  load_chunk(x, a.a_limbs + (size_t)s.v * n + (size_t)g.gl * K);
+  load_value<K, L>(g, a.a_limbs, s, x);
   store_result(a, s, g.gl, act == FPA_A_NEG ? (ea ^ 1u) : act, x);
 }
 
 /* ----------------------------- add / sub ----------------------------- */
-/* lds: >= n+1 words of this value's scratch */
+/* (re, R) = a +- b for the group's values (ea, A), (eb, B); lds: >= n+1 words */
 template <int K, int L>
-__device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, bool is_sub) {
+__device__ void core_addsub(const Group<L> &g, uint32_t *lds, int mode, bool is_sub, uint32_t ea,
+                            const uint32_t (&A)[K], uint32_t eb, const uint32_t (&B)[K], uint32_t &re,
+                            uint32_t (&R)[K]) {
   constexpr int n = K * L;
-  const int mode = (int)a.mode;
-  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
   uint32_t cls = fpa_classify(ea) * 9 + fpa_classify(eb);
   uint32_t act = is_sub ? k_sub_tab[cls] : k_add_tab[cls];
-  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
 
   if (act != FPA_A_ADD && act != FPA_A_SUB) {
-    uint32_t x[K];
-    if (act == FPA_A_COPY_A) { load_chunk(x, a.a_limbs + off); store_result(a, s, g.gl, ea, x); }
-    else if (act == FPA_A_COPY_B) { load_chunk(x, a.b_limbs + off); store_result(a, s, g.gl, eb, x); }
-    else if (act == FPA_A_NEG_B) { load_chunk(x, a.b_limbs + off); store_result(a, s, g.gl, eb ^ 1u, x); }
-    else store_special<K>(a, s, g.gl, act);
+    if (act == FPA_A_COPY_A) { re = ea; chunk_select(R, true, A, A); }
+    else if (act == FPA_A_COPY_B) { re = eb; chunk_select(R, true, B, B); }
+    else if (act == FPA_A_NEG_B) { re = eb ^ 1u; chunk_select(R, true, B, B); }
+    else {
+      re = act;
+#pragma unroll
+      for (int i = 0; i < K; i++) R[i] = 0;
+    }
     return;
   }
   if (is_sub) eb ^= 1u;
 
-  uint32_t A[K], B[K];
-  load_chunk(A, a.a_limbs + off);
-  load_chunk(B, a.b_limbs + off);
   const int64_t Ea = ea >> 1, Eb = eb >> 1;
   const bool swap = Ea < Eb;
   uint32_t X[K], Y[K];
@@ -190,7 +194,12 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
       sign = (swap ? ea : eb) & 1u;
     }
     int sh = g.clz(T);
-    if (sh == 32 * n) { store_special<K>(a, s, g.gl, FPA_C_ZERO); return; }
+    if (sh == 32 * n) {
+      re = FPA_C_ZERO;
+#pragma unroll
+      for (int i = 0; i < K; i++) R[i] = 0;
+      return;
+    }
     if (sh) {
       lds_store(lds + g.gl * K, T);
       wsync();
@@ -199,7 +208,8 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
       wsync();
     }
     E = Ex - sh;
-    store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), T);
+    re = pack_exp(E, sign, 5, 0x7FFFFFFFll);
+    chunk_select(R, true, T, T);
     return;
   }
 
@@ -264,7 +274,17 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
     }
   }
   if (round_mantissa<K, L>(g, T, mode, sign, rw, sticky)) E += 1;
-  store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), T);
+  re = pack_exp(E, sign, 5, 0x7FFFFFFFll);
+  chunk_select(R, true, T, T);
+}
+
+template <int K, int L>
+__device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, bool is_sub) {
+  uint32_t A[K], B[K], R[K], re;
+  load_value<K, L>(g, a.a_limbs, s, A);
+  load_value<K, L>(g, a.b_limbs, s, B);
+  core_addsub<K, L>(g, lds, (int)a.mode, is_sub, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  store_result(a, s, g.gl, re, R);
 }
 
 /* -------------------------------- mul -------------------------------- */
@@ -322,33 +342,33 @@ __device__ __forceinline__ uint32_t add_carry_digits(uint32_t (&dg)[KD], uint64_
   return (uint32_t)c;
 }
 
-/* lds: 2 * L * KD words (64-bit low columns, then digits) */
+/* (re, R) = a * b; lds: 2 * L * KD words (64-bit low columns, then digits) */
 template <int K, int L>
-__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ void core_mul(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&Aw)[K],
+                         uint32_t eb, const uint32_t (&Bw)[K], uint32_t &re, uint32_t (&R)[K]) {
   constexpr int n = K * L;
   constexpr int KD = mul_digits_per_lane(K);
   constexpr int ND = KD * L;               /* digits per operand */
   constexpr uint32_t M = (1u << MUL_RADIX) - 1;
   static_assert(2 * L <= 64, "low and high chunk carries share one 64-bit mask");
   static_assert(ND < 4096, "column sums of 27-bit digits must stay below 2^64");
-  const int mode = (int)a.mode;
-  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
   uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
-  if (act != FPA_A_MUL) { store_special<K>(a, s, g.gl, act); return; }
-  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+  if (act != FPA_A_MUL) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = 0;
+    return;
+  }
 
   /* recode both operands into radix-2^27 digit chunks via LDS */
   uint32_t A[KD], B[KD];
   {
-    uint32_t t[K];
-    load_chunk(t, a.a_limbs + off);
-    lds_store(lds + g.gl * K, t);
+    lds_store(lds + g.gl * K, Aw);
     wsync();
 #pragma unroll
     for (int k = 0; k < KD; k++) A[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
     wsync();
-    load_chunk(t, a.b_limbs + off);
-    lds_store(lds + g.gl * K, t);
+    lds_store(lds + g.gl * K, Bw);
     wsync();
 #pragma unroll
     for (int k = 0; k < KD; k++) B[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
@@ -441,7 +461,17 @@ __device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g,
   const uint32_t sign = (ea ^ eb) & 1u;
   int64_t E = (int64_t)(ea >> 1) + (int64_t)(eb >> 1) - FPA_BIAS_LL - (full ? 0 : 1);
   if (round_mantissa<K, L>(g, H, mode, sign, rw, sticky)) E += 1;
-  store_result(a, s, g.gl, pack_exp(E, sign, 5, 0x7FFFFFFFll), H);
+  re = pack_exp(E, sign, 5, 0x7FFFFFFFll);
+  chunk_select(R, true, H, H);
+}
+
+template <int K, int L>
+__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  uint32_t A[K], B[K], R[K], re;
+  load_value<K, L>(g, a.a_limbs, s, A);
+  load_value<K, L>(g, a.b_limbs, s, B);
+  core_mul<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  store_result(a, s, g.gl, re, R);
 }
 
 /* Multiply-subtract step shared by div and sqrt:
@@ -465,18 +495,19 @@ __device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], i
 }
 
 /* -------------------------------- div -------------------------------- */
-/* lds: n+2 words (quotient digits) */
+/* (re, Qr) = a / b; lds: n+2 words (quotient digits) */
 template <int K, int L>
-__device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+                         uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
   constexpr int n = K * L;
-  const int mode = (int)a.mode;
-  uint32_t ea = a.a_exps[s.v], eb = a.b_exps[s.v];
   uint32_t act = k_div_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
-  if (act != FPA_A_DIV) { store_special<K>(a, s, g.gl, act); return; }
-  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
-  uint32_t A[K], B[K], R[K];
-  load_chunk(A, a.a_limbs + off);
-  load_chunk(B, a.b_limbs + off);
+  if (act != FPA_A_DIV) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) Qr[i] = 0;
+    return;
+  }
+  uint32_t R[K];
 
   const uint32_t sign = (ea ^ eb) & 1u;
   int64_t E = (int64_t)(ea >> 1) - (int64_t)(eb >> 1) + FPA_BIAS_LL;
@@ -532,22 +563,34 @@ __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g,
   }
   wsync();
   if (round_mantissa<K, L>(g, M, mode, sign, rw, sticky)) E += 1;
-  store_result(a, s, g.gl, pack_exp(E, sign, 10, 0x7FFFFFFEll), M);
+  re = pack_exp(E, sign, 10, 0x7FFFFFFEll);
+  chunk_select(Qr, true, M, M);
+}
+
+template <int K, int L>
+__device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  uint32_t A[K], B[K], R[K], re;
+  load_value<K, L>(g, a.a_limbs, s, A);
+  load_value<K, L>(g, a.b_limbs, s, B);
+  core_div<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  store_result(a, s, g.gl, re, R);
 }
 
 /* -------------------------------- sqrt -------------------------------- */
 /* lds: 2n+2 words: T = 2S (+ current digit) at [0, n+1), root digits at [n+1, 2n+1) */
 template <int K, int L>
-__device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+                          uint32_t &re, uint32_t (&Sr)[K]) {
   constexpr int n = K * L;
-  const int mode = (int)a.mode;
-  uint32_t ea = a.a_exps[s.v];
   uint32_t act = k_sqrt_tab[fpa_classify(ea)];
-  if (act != FPA_A_SQRT) { store_special<K>(a, s, g.gl, act); return; }
-  const size_t off = (size_t)s.v * n + (size_t)g.gl * K;
+  if (act != FPA_A_SQRT) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) Sr[i] = 0;
+    return;
+  }
   uint32_t *Tl = lds, *Sl = lds + n + 1;
-  uint32_t A[K], R[K];
-  load_chunk(A, a.a_limbs + off);
+  uint32_t R[K];
   int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
   uint32_t shift_in = 0;
   if (e & 1) {
@@ -649,7 +692,16 @@ __device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g
   wsync();
   int64_t E = er + FPA_BIAS_LL;
   if (round_mantissa<K, L>(g, S, mode, 0u, gt ? 0x80000000u : 0u, rem_nz)) E += 1;
-  store_result(a, s, g.gl, pack_exp(E, 0u, 0, 0x7FFFFFFFll), S);
+  re = pack_exp(E, 0u, 0, 0x7FFFFFFFll);
+  chunk_select(Sr, true, S, S);
+}
+
+template <int K, int L>
+__device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+  uint32_t A[K], R[K], re;
+  load_value<K, L>(g, a.a_limbs, s, A);
+  core_sqrt<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  store_result(a, s, g.gl, re, R);
 }
 
 }  // namespace fpa

@@ -201,3 +201,31 @@ int fpa_cpu_op(int op, fpa_cpu_s *r, fpa_cpu_s *a, fpa_cpu_s *b, int mode) {
 size_t fpa_cpu_struct_size(void) { return sizeof(fpa_cpu_s); }
 int fpa_cpu_max_threads(void) { return omp_get_max_threads(); }
 const char *fpa_cpu_mpfr_version(void) { return mpfr_get_version(); }
+
+/* MPFR gold for dot chains: acc <- RN(acc + RN(x_k * y_k)) from +0, chains in parallel.
+ * Also the config-4 CPU baseline (timed by the caller). */
+void mpfr_dot_array(uint32_t count, uint32_t precision, int mode, const uint32_t *lengths, const uint64_t *offsets,
+                    const uint32_t *x_exps, const uint32_t *x_limbs, const uint32_t *y_exps, const uint32_t *y_limbs,
+                    uint32_t *r_exps, uint32_t *r_limbs, int threads) {
+  int n = (int)(precision / 32);
+  mpfr_rnd_t rnd = rnd_of(mode);
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads)
+  {
+    wide_range();
+    mpfr_t acc, p, x, y;
+    mpfr_inits2((mpfr_prec_t)precision, acc, p, x, y, (mpfr_ptr)0);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t j = 0; j < (int64_t)count; j++) {
+      mpfr_set_zero(acc, 1);
+      for (uint64_t k = offsets[j]; k < offsets[j] + lengths[j]; k++) {
+        mpf_from_fpa(x, x_exps[k], x_limbs + k * n, n);
+        mpf_from_fpa(y, y_exps[k], y_limbs + k * n, n);
+        mpfr_mul(p, x, y, rnd);
+        mpfr_add(acc, acc, p, rnd);
+      }
+      mpf_to_fpa(&r_exps[j], r_limbs + (size_t)j * n, n, acc, 5, 0x7FFFFFFFll);
+    }
+    mpfr_clears(acc, p, x, y, (mpfr_ptr)0);
+  }
+}

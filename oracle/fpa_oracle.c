@@ -496,3 +496,26 @@ void orc_bn_divmod(uint32_t *q, uint32_t *rem, const uint32_t *u, int nu, const 
 }
 int orc_bn_isqrt(uint32_t *r, const uint32_t *x, int nx) { return bn_isqrt(r, x, nx); }
 void orc_bn_mul(uint32_t *r, const uint32_t *a, int na, const uint32_t *b, int nb) { bn_mul(r, a, na, b, nb); }
+
+/* Dot chains: r_j = fold over k of acc <- RN(acc + RN(x_k * y_k)) from an exact zero,
+ * the same per-element op sequence MPFR gold uses (mpfr_mul then mpfr_add). */
+void orc_dot_array(uint32_t count, uint32_t precision, int mode, const uint32_t *lengths,
+                   const uint32_t *x_exps, const uint32_t *x_limbs, const uint32_t *y_exps,
+                   const uint32_t *y_limbs, uint32_t *r_exps, uint32_t *r_limbs) {
+  int n = (int)(precision / 32);
+  uint32_t *p = (uint32_t *)malloc(4 * (size_t)n), *acc = (uint32_t *)malloc(4 * (size_t)n);
+  uint64_t k = 0;
+  for (uint32_t j = 0; j < count; j++) {
+    uint32_t acc_e = C_ZERO, pe;
+    memset(acc, 0, 4 * (size_t)n);
+    for (uint32_t t = 0; t < lengths[j]; t++, k++) {
+      orc_mul(&pe, p, x_exps[k], x_limbs + k * n, y_exps[k], y_limbs + k * n, n, mode);
+      orc_addsub(&acc_e, acc, acc_e, acc, pe, p, n, mode, 0);
+    }
+    r_exps[j] = acc_e;
+    memcpy(r_limbs + (size_t)j * n, acc, 4 * (size_t)n);
+    if (acc_e <= C_SNAN) memset(r_limbs + (size_t)j * n, 0, 4 * (size_t)n);
+  }
+  free(p);
+  free(acc);
+}

@@ -202,3 +202,34 @@ def ref_generate(name, count, precision):
             z = out[k + "_exps"] == 2
             out[k + "_limbs"].reshape(-1, n)[z] = 0
     return out
+
+
+def dot(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode="rne"):
+    """Oracle dot chains (acc <- RN(acc + RN(x*y)) in index order); returns (r_exps, r_limbs)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    count = len(lengths)
+    n = precision // 32
+    r_exps = np.zeros(count, np.uint32)
+    r_limbs = np.zeros(count * n, np.uint32)
+    f = lib().orc_dot_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]
+    f(count, precision, MODES[mode], _p(lengths), _p(_c(x_exps)), _p(_c(x_limbs)), _p(_c(y_exps)),
+      _p(_c(y_limbs)), _p(r_exps), _p(r_limbs))
+    return r_exps, r_limbs
+
+
+def mpfr_dot(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode="rne", threads=0):
+    """MPFR gold for dot chains (the same mul-then-add sequence); returns (r_exps, r_limbs)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    count = len(lengths)
+    offs = np.zeros(count + 1, np.uint64)
+    offs[1:] = np.cumsum(lengths, dtype=np.uint64)
+    n = precision // 32
+    r_exps = np.zeros(count, np.uint32)
+    r_limbs = np.zeros(count * n, np.uint32)
+    f = lib().mpfr_dot_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.POINTER(ctypes.c_uint64),
+                  _u32p, _u32p, _u32p, _u32p, _u32p, _u32p, ctypes.c_int]
+    f(count, precision, MODES[mode], _p(lengths), offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+      _p(_c(x_exps)), _p(_c(x_limbs)), _p(_c(y_exps)), _p(_c(y_limbs)), _p(r_exps), _p(r_limbs), threads)
+    return r_exps, r_limbs

@@ -1,0 +1,115 @@
+"""Batched mixed-precision launches (fpa_batch.h) and dot chains: parity with the oracle
+and with the plain per-op launches; graph replays are each checked."""
+import numpy as np
+import pytest
+
+import cubigfloat_amd as F
+from cubigfloat_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+INV = {v: k for k, v in F.OPS.items()}
+MODE_NAMES = ["rz", "ru", "rd", "rne", "rna"]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def oracle_packed(ops, precs, modes, ae, al, be, bl):
+    off = W.limb_offsets(precs)
+    re = np.zeros(len(ops), np.uint32)
+    rl = np.zeros(int(off[-1]), np.uint32)
+    for op, p, m, idx, e1, l1, pos in W.split_by_op(ops, precs, modes, ae, al):
+        _, _, _, _, e2, l2, _ = next(x for x in W.split_by_op(ops, precs, modes, be, bl) if x[:3] == (op, p, m))
+        unary = INV[op] == "sqrt"
+        oe, ol = O.op(INV[op], p, e1, l1, None if unary else e2, None if unary else l2, mode=MODE_NAMES[m])
+        re[idx] = oe
+        rl[pos] = ol
+    return re, rl
+
+
+def compare_packed(precs, ra, la, rb, lb):
+    off = W.limb_offsets(precs)
+    bad = []
+    for i in range(len(precs)):
+        n = int(precs[i]) // 32
+        if ra[i] != rb[i] or (ra[i] > 8 and not np.array_equal(la[off[i]:off[i] + n], lb[off[i]:off[i] + n])):
+            bad.append(i)
+    return bad
+
+
+def test_batch_mixed_ops_precisions_modes():
+    ops, precs, modes = W.mixed_ops(3000, 1, precisions=F.BATCH_PRECISIONS, modes=(0, 1, 2, 3, 4))
+    # keep 16384-bit ops few: the oracle is slow there
+    keep = (precs < 16384) | (np.arange(len(precs)) % 16 == 0)
+    ops, precs, modes = ops[keep], precs[keep], modes[keep]
+    ae, al, be, bl = W.mixed_operands(ops, precs, 2)
+    say(f"[gpu] batch nops={len(ops)}")
+    B = F.Batch(ops, precs, modes)
+    say(f"[gpu] batch plan {B.info()}")
+    B.set_operands(ae, al, be, bl)
+    oe, ol = oracle_packed(ops, precs, modes, ae, al, be, bl)
+    for rep in range(3):
+        assert B.run() == F.SUCCESS
+        re, rl = B.results()
+        bad = compare_packed(precs, re, rl, oe, ol)
+        assert not bad, f"replay {rep}: {len(bad)} mismatches, first op {bad[0]}: op={INV[int(ops[bad[0]])]} p={precs[bad[0]]}"
+    assert B.run_plain() == F.SUCCESS
+    re, rl = B.results()
+    assert not compare_packed(precs, re, rl, oe, ol)
+    B.close()
+
+
+def test_batch_config3_distribution_matches_plain():
+    ops, precs, modes = W.mixed_ops(1 << 14, 3)
+    ae, al, be, bl = W.mixed_operands(ops, precs, 4)
+    say(f"[gpu] batch config-3 mix nops={len(ops)}")
+    B = F.Batch(ops, precs, modes)
+    B.set_operands(ae, al, be, bl)
+    B.run_plain()
+    pe, pl = B.results()
+    for rep in range(2):
+        B.run()
+        re, rl = B.results()
+        assert not compare_packed(precs, re, rl, pe, pl), f"replay {rep}"
+    # a sample against the oracle
+    sel = np.arange(0, len(ops), 37)
+    sub_ops, sub_p, sub_m = ops[sel], precs[sel], modes[sel]
+    off = W.limb_offsets(precs)
+    def take(e, l):
+        return e[sel], np.concatenate([l[off[i]:off[i + 1]] for i in sel])
+    sae, sal = take(ae, al)
+    sbe, sbl = take(be, bl)
+    oe, ol = oracle_packed(sub_ops, sub_p, sub_m, sae, sal, sbe, sbl)
+    sre, srl = take(re, rl)
+    assert not compare_packed(sub_p, sre, srl, oe, ol)
+    B.close()
+
+
+def test_batch_rejects_bad_precision():
+    with pytest.raises(ValueError):
+        F.Batch([F.OPS["mul"]], [3072])
+
+
+@pytest.mark.parametrize("p", [256, 2048, 4096])
+def test_dot_chains_vs_oracle(p):
+    rng = np.random.default_rng(p)
+    lengths = rng.integers(0, 40, 150).astype(np.uint32)
+    lengths[:3] = [0, 1, 64]
+    total = int(lengths.sum())
+    xe, xl, ye, yl = F.generate(2, 9, "mul", p, total)
+    for mode in ("rne", "rz", "rd"):
+        say(f"[gpu] dot p={p} chains={len(lengths)} mode={mode}")
+        re, rl = F.dot_chains(p, lengths, xe, xl, ye, yl, mode=F.MODES[mode])
+        oe, ol = O.dot(p, lengths, xe, xl, ye, yl, mode=mode)
+        eq = O.results_equal(re, rl, oe, ol, p)
+        assert eq.all(), f"{int((~eq).sum())} mismatches"
+
+
+def test_dot_config4_shape_sample():
+    lengths, xe, xl, ye, yl = W.dot_workload(256, 2048, 11)
+    say(f"[gpu] dot config-4 sample chains=256 elements={int(lengths.sum())}")
+    re, rl = F.dot_chains(2048, lengths, xe, xl, ye, yl)
+    me, ml = O.mpfr_dot(2048, lengths, xe, xl, ye, yl)
+    assert O.results_equal(re, rl, me, ml, 2048).all()

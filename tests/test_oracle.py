@@ -226,3 +226,17 @@ def test_bignum_kernels_vs_python():
         import math
         assert _to_int(s) == math.isqrt(X)
         assert bool(inexact) == (math.isqrt(X) ** 2 != X)
+
+
+@pytest.mark.parametrize("p", [256, 2048])
+def test_oracle_dot_vs_mpfr(p):
+    """Dot chains: the oracle's fold of exact mul/add equals MPFR's mpfr_mul/mpfr_add fold."""
+    import cubigfloat_amd as F
+    rng = np.random.default_rng(p)
+    lengths = rng.integers(0, 30, 60).astype(np.uint32)
+    total = int(lengths.sum())
+    xe, xl, ye, yl = F.generate(2, 5, "mul", p, total)
+    for mode in ("rne", "rz", "ru"):
+        oe, ol = O.dot(p, lengths, xe, xl, ye, yl, mode=mode)
+        me, ml = O.mpfr_dot(p, lengths, xe, xl, ye, yl, mode=mode)
+        assert O.results_equal(oe, ol, me, ml, p).all()
