@@ -320,3 +320,105 @@ def dot_chains(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode=ROUND_
     finally:
         for a in (X, Y, R):
             a.close()
+
+
+# ------------------------------ server (fpa_serve.h) ------------------------------
+class Request(ctypes.Structure):
+    """fpa_request (fpa_serve.h)."""
+    _fields_ = [("nops", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("ops", ctypes.c_void_p), ("modes", ctypes.c_void_p),
+                ("a_exps", ctypes.c_void_p), ("a_limbs", ctypes.c_void_p),
+                ("b_exps", ctypes.c_void_p), ("b_limbs", ctypes.c_void_p),
+                ("r_exps", ctypes.c_void_p), ("r_limbs", ctypes.c_void_p),
+                ("arrival", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("status", ctypes.c_int32), ("slot", ctypes.c_uint32)]
+
+
+lib.fpa_server_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t, ctypes.c_uint32, ctypes.c_size_t]
+lib.fpa_server_destroy.argtypes = [_vp]
+lib.fpa_server_submit.argtypes = [_vp, ctypes.POINTER(Request)]
+lib.fpa_server_wait.argtypes = [_vp, ctypes.POINTER(Request)]
+lib.fpa_server_run_trace.argtypes = [_vp, ctypes.POINTER(Request), ctypes.c_uint32]
+lib.fpa_serve_plain_trace.argtypes = [ctypes.POINTER(Request), ctypes.c_uint32]
+lib.fpa_request_bytes.argtypes = [ctypes.POINTER(Request)]
+lib.fpa_request_bytes.restype = ctypes.c_size_t
+lib.fpa_server_peak_bytes.argtypes = [_vp]
+lib.fpa_server_peak_bytes.restype = ctypes.c_size_t
+lib.fpa_server_budget_waits.argtypes = [_vp]
+lib.fpa_server_budget_waits.restype = ctypes.c_uint64
+
+
+class Trace:
+    """A list of requests with host buffers kept alive for the C server.
+
+    Each request: dict(precision, ops (uint8), modes (uint8), a_exps, a_limbs, b_exps,
+    b_limbs, arrival)."""
+
+    def __init__(self, reqs):
+        self.reqs = reqs
+        self.arr = (Request * len(reqs))()
+        self.keep = []
+        for i, q in enumerate(reqs):
+            n = q["precision"] // 32
+            nops = len(q["ops"])
+            bufs = {k: np.ascontiguousarray(q[k], dtype=np.uint8 if k in ("ops", "modes") else np.uint32)
+                    for k in ("ops", "modes", "a_exps", "a_limbs", "b_exps", "b_limbs")}
+            bufs["r_exps"] = np.zeros(nops, np.uint32)
+            bufs["r_limbs"] = np.zeros(nops * n, np.uint32)
+            self.keep.append(bufs)
+            r = self.arr[i]
+            r.nops, r.precision, r.arrival = nops, q["precision"], float(q["arrival"])
+            for k, v in bufs.items():
+                setattr(r, k, v.ctypes.data)
+
+    def __len__(self):
+        return len(self.reqs)
+
+    def results(self, i):
+        return self.keep[i]["r_exps"], self.keep[i]["r_limbs"]
+
+    def times(self):
+        a = np.array([r.arrival for r in self.arr])
+        s = np.array([r.t_admit for r in self.arr])
+        d = np.array([r.t_done for r in self.arr])
+        st = np.array([r.status for r in self.arr])
+        return a, s, d, st
+
+    def reset(self):
+        for r in self.arr:
+            r.status, r.t_admit, r.t_done = -1, 0.0, 0.0
+        for b in self.keep:
+            b["r_exps"][:] = 0
+            b["r_limbs"][:] = 0
+
+
+class Server:
+    """fpa_server_t: resident pool of budget_bytes, `slots` streams."""
+
+    def __init__(self, budget_bytes, slots=8, max_request_bytes=0):
+        self.h = _vp()
+        rc = lib.fpa_server_create(ctypes.byref(self.h), budget_bytes, slots, max_request_bytes)
+        if rc != SUCCESS:
+            raise ValueError(f"fpa_server_create returned {rc}")
+
+    def run_trace(self, trace):
+        return lib.fpa_server_run_trace(self.h, trace.arr, len(trace))
+
+    def peak_bytes(self):
+        return lib.fpa_server_peak_bytes(self.h)
+
+    def budget_waits(self):
+        return lib.fpa_server_budget_waits(self.h)
+
+    def close(self):
+        if self.h:
+            lib.fpa_server_destroy(self.h)
+            self.h = None
+
+
+def serve_plain(trace):
+    return lib.fpa_serve_plain_trace(trace.arr, len(trace))
+
+
+def request_bytes(trace, i):
+    return lib.fpa_request_bytes(ctypes.byref(trace.arr[i]))

@@ -84,3 +84,26 @@ def dot_workload(nchains, precision, seed, lmin=8, lmax=512):
         ex = rng.integers(-32, 33, total)
         e[:] = (2 * (BIAS + ex) + sign).astype(np.uint32)
     return lengths, xe, xl, ye, yl
+
+
+SERVE_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192, 16384)
+
+
+def served_requests(nreq, seed, rate, max_ops=4096, precisions=SERVE_PRECISIONS, mix=CONFIG3_MIX):
+    """Poisson arrivals at `rate` requests/s; request size log-uniform in [1, max_ops];
+    one precision per request, uniform over the (power-of-two) class list; op mix as the
+    mixed batch.  Returns a list of request dicts for cubigfloat_amd.Trace."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        nops = int(np.floor(np.exp(rng.uniform(0, np.log(max_ops + 1)))))
+        nops = min(max(nops, 1), max_ops)
+        p = int(precisions[rng.integers(0, len(precisions))])
+        ops, _, modes = mixed_ops(nops, seed * 7919 + i, precisions=(p,), mix=mix)
+        precs = np.full(nops, p, np.uint32)
+        ae, al, be, bl = mixed_operands(ops, precs, seed * 104729 + i)
+        reqs.append({"precision": p, "ops": ops, "modes": modes, "a_exps": ae, "a_limbs": al,
+                     "b_exps": be, "b_limbs": bl, "arrival": t})
+    return reqs
