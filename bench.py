@@ -133,6 +133,150 @@ def cpu_baseline(op, p, sample_in, gpu_out, threads):
     return cnt * reps / t, reps * cnt, bool(ok.all()), int((~ok).sum())
 
 
+def pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def bench_config3(F, hip, nops, steps, cpu, threads):
+    """2^18 mixed ops (precision log-uniform over 256..8192, 40% mul / 15% add / 15% sub /
+    20% div / 10% sqrt, config-2 operands) in one graph launch (fork/join of per-segment
+    kernels) vs the same kernels as plain launches on one stream, MPFR on host cores."""
+    from cubigfloat_amd import workloads as W
+    ops, precs, modes = W.mixed_ops(nops, 3)
+    ae, al, be, bl = W.mixed_operands(ops, precs, 3)
+    B = F.Batch(ops, precs, modes)
+    B.set_operands(ae, al, be, bl)
+    res = {"workload": f"{nops} mixed ops, precision log-uniform over 256..8192, op mix 40/15/15/20/10",
+           "plan": B.info()}
+    for name, fn in (("batch_graph", B.run), ("plain_per_op", B.run_plain)):
+        fn()
+        hip.sync()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0)
+        for _ in range(steps):
+            fn()
+        hip.record(e1)
+        ms = hip.elapsed_ms(e0, e1) / steps
+        res[name] = {"ms": ms, "ops_per_s": nops / (ms / 1e3)}
+        if name == "batch_graph":
+            be_, bl_ = B.results()
+        else:
+            pe_, pl_ = B.results()
+    off = W.limb_offsets(precs)
+    same = bool((be_ == pe_).all() and (bl_ == pl_).all())
+    res["batch_equals_plain"] = same
+    if cpu:
+        from oracle import oracle as O
+        t_cpu, sample_ok = 0.0, True
+        O.cpu_set_threads(threads)
+        inv = {v: k for k, v in F.OPS.items()}
+        for op, p, m, idx, e1, l1, pos in W.split_by_op(ops, precs, modes, ae, al):
+            e2, l2 = be[idx], bl[pos]
+            cnt = len(idx)
+            A, R = O.CpuArray(cnt, p), O.CpuArray(cnt, p)
+            A.load(e1, l1)
+            Bc = None
+            if inv[op] != "sqrt":
+                Bc = O.CpuArray(cnt, p)
+                Bc.load(e2, l2)
+            t0 = time.perf_counter()
+            O.cpu_op(inv[op], R, A, Bc, ["rz", "ru", "rd", "rne", "rna"][m])
+            t_cpu += time.perf_counter() - t0
+            ce, cl = R.store(inv[op])
+            sample_ok &= bool(O.results_equal(be_[idx], bl_[pos], ce, cl, p).all())
+            for x in (A, Bc, R):
+                if x is not None:
+                    x.close()
+        res["cpu_mpfr"] = {"ops_per_s": nops / t_cpu, "cores": threads}
+        res["bit_exact_vs_mpfr"] = sample_ok
+    B.close()
+    return res
+
+
+def bench_config4(F, hip, nchains, steps, cpu, threads):
+    """16,384 dot products at 2048 bits, lengths U[8,512], per-element RN mul then RN add
+    accumulated in registers, one launch."""
+    from cubigfloat_amd import workloads as W
+    lengths, xe, xl, ye, yl = W.dot_workload(nchains, 2048, 4)
+    total = int(lengths.sum())
+    X, Y, R = F.FpaArray(total, 2048), F.FpaArray(total, 2048), F.FpaArray(nchains, 2048)
+    X.upload(xe, xl)
+    Y.upload(ye, yl)
+    plan = F.DotPlan(lengths)
+    plan.run(R, X, Y)
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0)
+    for _ in range(steps):
+        plan.run(R, X, Y)
+    hip.record(e1)
+    ms = hip.elapsed_ms(e0, e1) / steps
+    re, rl = R.download()
+    res = {"workload": f"{nchains} dot products at 2048 bits, lengths U[8,512], {total} elements",
+           "ms": ms, "dots_per_s": nchains / (ms / 1e3), "mul_add_pairs_per_s": total / (ms / 1e3)}
+    if cpu:
+        from oracle import oracle as O
+        k = 512
+        off = np.concatenate([[0], np.cumsum(lengths[:k], dtype=np.int64)])
+        n = 64
+        t0 = time.perf_counter()
+        me, ml = O.mpfr_dot(2048, lengths[:k], xe[:off[-1]], xl[:off[-1] * n], ye[:off[-1]], yl[:off[-1] * n],
+                            threads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_mpfr"] = {"dots_per_s": k / dt, "cores": threads, "sample": f"first {k} chains"}
+        res["bit_exact_vs_mpfr"] = bool(O.results_equal(re[:k], rl[:k * n], me, ml, 2048).all())
+    for a in (X, Y, R):
+        a.close()
+    plan.close()
+    return res
+
+
+def bench_config5(F, nreq, budget):
+    """Served workload: Poisson arrivals, requests of 1..4096 ops (log-uniform) at one
+    precision each (256..16384), op mix of the mixed batch; continuous-batching server
+    (resident pool = budget, 8 streams) vs plain per-request launches."""
+    from cubigfloat_amd import workloads as W
+    reqs = W.served_requests(nreq, 5, rate=0.0)
+    nops = sum(len(q["ops"]) for q in reqs)
+
+    def run(kind, rate):
+        rng = np.random.default_rng(17)
+        t = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate > 0 else np.zeros(len(reqs))
+        tr = F.Trace(reqs)
+        for i, q in enumerate(tr.arr):
+            q.arrival = float(t[i])
+        if kind == "server":
+            srv = F.Server(budget, slots=8, max_request_bytes=64 << 20)
+            rc = srv.run_trace(tr)
+            extra = {"peak_pool_bytes": srv.peak_bytes(), "budget_waits": srv.budget_waits()}
+            srv.close()
+        else:
+            rc = F.serve_plain(tr)
+            extra = {}
+        a, s, d, st = tr.times()
+        assert rc == F.SUCCESS and (st == 0).all()
+        lat = (d - a) * 1e3
+        span = d.max() - a.min()
+        out = {"rate_req_per_s": rate, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+               "mean_ms": float(lat.mean()), "ops_per_s": nops / span, "req_per_s": len(reqs) / span}
+        out.update(extra)
+        return out, tr
+
+    cap_plain, tp = run("plain", 0.0)
+    cap_server, ts = run("server", 0.0)
+    same = all((tp.results(i)[0] == ts.results(i)[0]).all() and (tp.results(i)[1] == ts.results(i)[1]).all()
+               for i in range(len(reqs)))
+    res = {"workload": f"{nreq} requests, {nops} ops, sizes log-uniform 1..4096, precisions 256..16384",
+           "budget_bytes": budget, "slots": 8, "capacity_plain": cap_plain, "capacity_server": cap_server,
+           "server_results_equal_plain": same, "loads": []}
+    for frac in (0.5, 0.8):
+        lam = frac * cap_plain["req_per_s"]
+        ps, _ = run("plain", lam)
+        ss, _ = run("server", lam)
+        res["loads"].append({"fraction_of_plain_capacity": frac, "lambda_req_per_s": lam, "plain": ps, "server": ss})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +285,7 @@ def main():
     ap.add_argument("--count", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and MPFR checks")
     ap.add_argument("--only", default="", help="comma list of extra ops to time (default all)")
+    ap.add_argument("--configs", default="3,4,5", help="batched/chained/served configurations to run ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -192,6 +337,18 @@ def main():
                          f"elements of the same workload, repeated >=1 s; {done} muls",
                "single_core_mul_per_s": rate_1, "gpu_sample_bit_exact_vs_mpfr": ok}
 
+    configs = {}
+    wanted_cfg = set(c for c in args.configs.split(",") if c)
+    if "3" in wanted_cfg:
+        print("[bench] config 3: mixed batch, one graph launch", file=sys.stderr, flush=True)
+        configs["config3"] = bench_config3(F, hip, 1 << 18, 5, not args.no_cpu, threads)
+    if "4" in wanted_cfg:
+        print("[bench] config 4: dot chains", file=sys.stderr, flush=True)
+        configs["config4"] = bench_config4(F, hip, 16384, 3, not args.no_cpu, threads)
+    if "5" in wanted_cfg:
+        print("[bench] config 5: served workload", file=sys.stderr, flush=True)
+        configs["config5"] = bench_config5(F, 400, 4 << 30)
+
     ops = head["ops_per_s"]
     res = {
         "metric": "4096-bit correctly-rounded mul/s", "value": ops, "unit": "mul/s",
@@ -210,6 +367,7 @@ def main():
         "cpu_baseline": cpu,
         "checks_bit_exact_vs_mpfr": checks,
         "extras": extras,
+        "configs": configs,
         "wall_s": head["wall_s"],
     }
     print(json.dumps(res), flush=True)

@@ -217,13 +217,17 @@ _vp = ctypes.c_void_p
 lib.fpa_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8), _u32p,
                                  ctypes.POINTER(ctypes.c_uint8)]
 lib.fpa_batch_set_operands.argtypes = [_vp, _u32p, _u32p, _u32p, _u32p]
+lib.fpa_batch_create_branches.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8),
+                                          _u32p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32]
 lib.fpa_batch_run.argtypes = [_vp, _vp]
 lib.fpa_batch_run_plain.argtypes = [_vp, _vp]
 lib.fpa_batch_get_results.argtypes = [_vp, _vp, _u32p, _u32p]
 lib.fpa_batch_destroy.argtypes = [_vp]
-for _n in ("fpa_batch_segments", "fpa_batch_tiles", "fpa_batch_grid"):
+for _n in ("fpa_batch_segments", "fpa_batch_branches"):
     getattr(lib, _n).argtypes = [_vp]
     getattr(lib, _n).restype = ctypes.c_uint32
+lib.fpa_batch_blocks.argtypes = [_vp]
+lib.fpa_batch_blocks.restype = ctypes.c_uint64
 lib.fpa_batch_device_bytes.argtypes = [_vp]
 lib.fpa_batch_device_bytes.restype = ctypes.c_size_t
 lib.fpa_batch_total_limbs.argtypes = [_vp]
@@ -238,15 +242,20 @@ BATCH_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192, 16384)
 
 
 class Batch:
-    """A planned mixed-precision batch (fpa_batch_t): one captured persistent launch."""
+    """A planned mixed-precision batch (fpa_batch_t): one captured fork/join graph."""
 
-    def __init__(self, ops, precisions, modes=None):
+    def __init__(self, ops, precisions, modes=None, branches=None):
         ops = np.ascontiguousarray([OPS[o] if isinstance(o, str) else o for o in ops], dtype=np.uint8)
         precisions = np.ascontiguousarray(precisions, dtype=np.uint32)
         modes = np.ascontiguousarray(np.full(len(ops), ROUND_TIES_TO_EVEN) if modes is None else modes, dtype=np.uint8)
         self.h = _vp()
-        rc = lib.fpa_batch_create(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                                  _ptr(precisions), modes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        u8 = ctypes.POINTER(ctypes.c_uint8)
+        if branches is None:
+            rc = lib.fpa_batch_create(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(u8), _ptr(precisions),
+                                      modes.ctypes.data_as(u8))
+        else:
+            rc = lib.fpa_batch_create_branches(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(u8),
+                                               _ptr(precisions), modes.ctypes.data_as(u8), branches)
         if rc != SUCCESS:
             raise ValueError(f"fpa_batch_create returned {rc}")
         self.ops, self.precisions, self.nops = ops, precisions, len(ops)
@@ -272,8 +281,8 @@ class Batch:
         return e, l
 
     def info(self):
-        return {"segments": lib.fpa_batch_segments(self.h), "tiles": lib.fpa_batch_tiles(self.h),
-                "grid": lib.fpa_batch_grid(self.h), "device_bytes": lib.fpa_batch_device_bytes(self.h)}
+        return {"segments": lib.fpa_batch_segments(self.h), "branches": lib.fpa_batch_branches(self.h),
+                "blocks": lib.fpa_batch_blocks(self.h), "device_bytes": lib.fpa_batch_device_bytes(self.h)}
 
     def close(self):
         if self.h:

@@ -1,6 +1,14 @@
 /*
- * Host side of fpa_batch.h: batch planning (segments, operand pools, cost-sorted tiles,
- * hipGraph capture of the persistent kernel) and dot-chain plans.
+ * Host side of fpa_batch.h: batch planning (segments, one operand/result pool, hipGraph
+ * capture of a fork/join graph of per-segment kernels) and dot-chain plans.
+ *
+ * A batch becomes one graph: a fork node, `branches` independent chains of kernel nodes
+ * and a join.  Every kernel node is the element-wise kernel of one (op, precision, mode)
+ * segment, launched with the dispatcher's shape for that (op, precision).  Segments are
+ * dealt to the branches longest-processing-time first (estimated cost = values x per-value
+ * cost of the shape), so the branches end at about the same time and the short segments
+ * fill the gaps the long ones leave on the chip.  Nothing in the graph needs resetting
+ * between replays.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,65 +28,69 @@
     }                                                                                      \
   } while (0)
 
-int fpa_impl_batch_kind(int op, uint32_t precision);
-int fpa_impl_batch_kind_shape(int kind, int *op, uint32_t *precision, int *K, int *L);
-int fpa_impl_batch_blocks_per_cu(void);
-int fpa_impl_batch_launch(const fpa_batch_segment *segs, const void *tiles, uint32_t ntiles, uint32_t grid,
-                          void *stream);
+int fpa_impl_batch_supported(int op, uint32_t precision);
 int fpa_impl_dot_launch(const fpa_dot_arguments *d, void *stream);
 void *fpa_get_stream(void);
 
+#define FPA_BATCH_MAX_BRANCHES 8
+
 typedef struct {
-  int op, mode, kind, K, L;
-  uint32_t precision, count;
+  int op, mode, K, L, variant, branch;
+  uint32_t precision, count, blocks;
+  double cost;
   size_t off[6];           /* pool offsets: r_exps, r_limbs, a_exps, a_limbs, b_exps, b_limbs */
+  fpa_arguments args;      /* device pointers into the pool */
 } seg_host;
 
-typedef struct { uint32_t seg, first; float cost; } tile_host;
-
 struct fpa_batch_s {
-  uint32_t nops, nseg, ntiles, grid;
+  uint32_t nops, nseg, branches;
+  uint64_t blocks;
   uint32_t *op_seg, *op_slot;
   uint64_t *op_off;        /* packed host limb offset of each op */
   uint64_t total_limbs;
   seg_host *sh;
+  uint32_t *order;         /* segments, most expensive first */
   uint8_t *pool;           /* device */
   size_t pool_bytes;
   uint8_t *mirror;         /* pinned host copy of the pool layout */
-  fpa_batch_segment *segs_dev;
-  uint32_t *tiles_dev;     /* ntiles x (seg, first) */
   hipGraph_t graph;
   hipGraphExec_t exec;
 };
 
-int fpa_batch_precision_supported(uint32_t precision) { return fpa_impl_batch_kind(FPA_OP_MUL, precision) >= 0; }
+int fpa_batch_precision_supported(uint32_t precision) { return fpa_precision_supported(precision); }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-/* relative time one wavefront spends on one tile of a kind (orders tiles, longest first) */
-static float tile_cost(int op, int K, int L) {
+/* relative time one wavefront spends on its 64/L values of a segment */
+static double wave_cost(int op, int K, int L) {
   int n = K * L, KD = (32 * K + 26) / 27;
   switch (op) {
-    case FPA_OP_MUL: return (float)L * KD * KD + 40.0f * KD;
-    case FPA_OP_DIV: return (float)(n + 1) * (3 * K + 40);
-    case FPA_OP_SQRT: return (float)n * (3 * K + 50);
-    default: return 8.0f * K + 60.0f;
+    case FPA_OP_MUL: return (double)L * KD * KD + 40.0 * KD;
+    case FPA_OP_DIV: return (double)(n + 1) * (3 * K + 40);
+    case FPA_OP_SQRT: return (double)n * (3 * K + 50);
+    default: return 8.0 * K + 60.0;
   }
 }
 
-static int tile_cmp(const void *x, const void *y) {
-  const tile_host *a = (const tile_host *)x, *b = (const tile_host *)y;
-  if (a->cost != b->cost) return a->cost > b->cost ? -1 : 1;
-  if (a->seg != b->seg) return a->seg < b->seg ? -1 : 1;
-  return a->first < b->first ? -1 : (a->first > b->first);
+static const seg_host *sort_base;
+static int seg_cmp(const void *x, const void *y) {
+  uint32_t i = *(const uint32_t *)x, j = *(const uint32_t *)y;
+  double a = sort_base[i].cost, b = sort_base[j].cost;
+  if (a != b) return a > b ? -1 : 1;
+  return i < j ? -1 : (i > j);
 }
 
-fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops, const uint32_t *precisions,
-                            const uint8_t *modes) {
+static int launch_segment(const seg_host *g, void *stream) {
+  return fpa_kernel_launch(g->op, &g->args, g->variant, stream);
+}
+
+fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
+                                     const uint32_t *precisions, const uint8_t *modes, uint32_t branches) {
   *out = NULL;
+  if (branches < 1 || branches > FPA_BATCH_MAX_BRANCHES) return fpa_invalid_counts;
   for (uint32_t i = 0; i < nops; i++) {
     if (ops[i] < FPA_OP_ADD || ops[i] > FPA_OP_SQRT || modes[i] > 4) return fpa_math_error;
-    if (fpa_impl_batch_kind(ops[i], precisions[i]) < 0) return fpa_invalid_precisions;
+    if (!fpa_batch_precision_supported(precisions[i])) return fpa_invalid_precisions;
   }
   struct fpa_batch_s *b = (struct fpa_batch_s *)calloc(1, sizeof *b);
   b->nops = nops;
@@ -99,10 +111,8 @@ fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
       b->sh[s].op = ops[i];
       b->sh[s].precision = precisions[i];
       b->sh[s].mode = modes[i];
-      b->sh[s].kind = fpa_impl_batch_kind(ops[i], precisions[i]);
-      int o;
-      uint32_t pp;
-      fpa_impl_batch_kind_shape(b->sh[s].kind, &o, &pp, &b->sh[s].K, &b->sh[s].L);
+      b->sh[s].variant = fpa_kernel_get_variant(ops[i], precisions[i]);
+      fpa_kernel_variant_shape(ops[i], precisions[i], b->sh[s].variant, &b->sh[s].K, &b->sh[s].L);
       b->nseg++;
     }
     b->op_seg[i] = s;
@@ -112,7 +122,7 @@ fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
   }
   b->op_off[nops] = off;
   b->total_limbs = off;
-  /* pool layout */
+  /* pool layout, then the kernel arguments of every segment */
   size_t pos = 0;
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
@@ -124,11 +134,9 @@ fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
   b->pool_bytes = pos ? pos : 256;
   FPA_HIP(hipMalloc((void **)&b->pool, b->pool_bytes));
   FPA_HIP(hipHostMalloc((void **)&b->mirror, b->pool_bytes, hipHostMallocDefault));
-  /* device segment descriptors */
-  fpa_batch_segment *sd = (fpa_batch_segment *)calloc(b->nseg ? b->nseg : 1, sizeof(fpa_batch_segment));
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
-    fpa_arguments *a = &sd[s].args;
+    fpa_arguments *a = &g->args;
     a->count = g->count;
     a->precision = g->precision;
     a->mode = (fpa_rounding_mode)g->mode;
@@ -138,47 +146,55 @@ fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
     a->a_limbs = (uint32_t *)(b->pool + g->off[3]);
     a->b_exps = (uint32_t *)(b->pool + (g->op == FPA_OP_SQRT ? g->off[2] : g->off[4]));
     a->b_limbs = (uint32_t *)(b->pool + (g->op == FPA_OP_SQRT ? g->off[3] : g->off[5]));
-    sd[s].kind = (uint32_t)g->kind;
+    uint32_t vpb = 4 * (64 / g->L);
+    g->blocks = (g->count + vpb - 1) / vpb;
+    b->blocks += g->blocks;
+    g->cost = (double)g->blocks * 4 * wave_cost(g->op, g->K, g->L);
   }
-  FPA_HIP(hipMalloc((void **)&b->segs_dev, sizeof(fpa_batch_segment) * (b->nseg ? b->nseg : 1)));
-  FPA_HIP(hipMemcpy(b->segs_dev, sd, sizeof(fpa_batch_segment) * b->nseg, hipMemcpyHostToDevice));
-  free(sd);
-  /* tiles: one wavefront's values of one segment, most expensive first */
-  uint64_t nt = 0;
-  for (uint32_t s = 0; s < b->nseg; s++) nt += (b->sh[s].count + (64 / b->sh[s].L) - 1) / (64 / b->sh[s].L);
-  tile_host *th = (tile_host *)malloc(sizeof(tile_host) * (nt ? nt : 1));
-  uint64_t k = 0;
-  for (uint32_t s = 0; s < b->nseg; s++) {
-    uint32_t vpw = 64 / b->sh[s].L;
-    float c = tile_cost(b->sh[s].op, b->sh[s].K, b->sh[s].L);
-    for (uint32_t f = 0; f < b->sh[s].count; f += vpw) { th[k].seg = s; th[k].first = f; th[k].cost = c; k++; }
+  /* longest processing time first onto the least-loaded branch */
+  b->order = (uint32_t *)malloc(sizeof(uint32_t) * (b->nseg ? b->nseg : 1));
+  for (uint32_t s = 0; s < b->nseg; s++) b->order[s] = s;
+  sort_base = b->sh;
+  qsort(b->order, b->nseg, sizeof(uint32_t), seg_cmp);
+  b->branches = branches < b->nseg ? branches : (b->nseg ? b->nseg : 1);
+  double load[FPA_BATCH_MAX_BRANCHES] = { 0 };
+  for (uint32_t k = 0; k < b->nseg; k++) {
+    seg_host *g = &b->sh[b->order[k]];
+    uint32_t best = 0;
+    for (uint32_t j = 1; j < b->branches; j++)
+      if (load[j] < load[best]) best = j;
+    g->branch = (int)best;
+    load[best] += g->cost;
   }
-  qsort(th, nt, sizeof(tile_host), tile_cmp);
-  uint32_t *tt = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (nt ? nt : 1));
-  for (uint64_t i = 0; i < nt; i++) { tt[2 * i] = th[i].seg; tt[2 * i + 1] = th[i].first; }
-  b->ntiles = (uint32_t)nt;
-  FPA_HIP(hipMalloc((void **)&b->tiles_dev, sizeof(uint32_t) * 2 * (nt ? nt : 1)));
-  FPA_HIP(hipMemcpy(b->tiles_dev, tt, sizeof(uint32_t) * 2 * nt, hipMemcpyHostToDevice));
-  free(th);
-  free(tt);
-  /* persistent grid: every resident block slot of the device, no more than the tiles need */
-  int dev = 0, cus = 0;
-  FPA_HIP(hipGetDevice(&dev));
-  FPA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  uint64_t grid = (uint64_t)cus * fpa_impl_batch_blocks_per_cu();
-  uint64_t need = (nt + 3) / 4;
-  b->grid = (uint32_t)(grid < need ? grid : (need ? need : 1));
-  /* capture one launch in a graph */
-  hipStream_t cs;
-  FPA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  FPA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  int rc = fpa_impl_batch_launch(b->segs_dev, b->tiles_dev, b->ntiles, b->grid, cs);
-  FPA_HIP(hipStreamEndCapture(cs, &b->graph));
-  if (rc != 0) { fprintf(stderr, "batch kernel capture failed: %d\n", rc); exit(1); }
+  /* capture: fork on an event, one stream per branch, join back onto the origin stream */
+  hipStream_t st[FPA_BATCH_MAX_BRANCHES];
+  hipEvent_t ev[FPA_BATCH_MAX_BRANCHES + 1];
+  for (uint32_t j = 0; j < b->branches; j++) FPA_HIP(hipStreamCreateWithFlags(&st[j], hipStreamNonBlocking));
+  for (uint32_t j = 0; j <= b->branches; j++) FPA_HIP(hipEventCreateWithFlags(&ev[j], hipEventDisableTiming));
+  FPA_HIP(hipStreamBeginCapture(st[0], hipStreamCaptureModeThreadLocal));
+  FPA_HIP(hipEventRecord(ev[b->branches], st[0]));
+  for (uint32_t j = 1; j < b->branches; j++) FPA_HIP(hipStreamWaitEvent(st[j], ev[b->branches], 0));
+  int rc = 0;
+  for (uint32_t k = 0; k < b->nseg && rc == 0; k++) {
+    seg_host *g = &b->sh[b->order[k]];
+    rc = launch_segment(g, st[g->branch]);
+  }
+  for (uint32_t j = 1; j < b->branches; j++) {
+    FPA_HIP(hipEventRecord(ev[j], st[j]));
+    FPA_HIP(hipStreamWaitEvent(st[0], ev[j], 0));
+  }
+  FPA_HIP(hipStreamEndCapture(st[0], &b->graph));
+  if (rc != 0) { fprintf(stderr, "batch graph capture failed: kernel launch returned %d\n", rc); exit(1); }
   FPA_HIP(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
-  FPA_HIP(hipStreamDestroy(cs));
+  for (uint32_t j = 0; j <= b->branches; j++) FPA_HIP(hipEventDestroy(ev[j]));
+  for (uint32_t j = 0; j < b->branches; j++) FPA_HIP(hipStreamDestroy(st[j]));
   *out = b;
   return fpa_success;
+}
+
+fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops, const uint32_t *precisions,
+                            const uint8_t *modes) {
+  return fpa_batch_create_branches(out, nops, ops, precisions, modes, FPA_BATCH_DEFAULT_BRANCHES);
 }
 
 static void scatter_operand(struct fpa_batch_s *b, int which, const uint32_t *exps, const uint32_t *limbs) {
@@ -195,13 +211,11 @@ static void scatter_operand(struct fpa_batch_s *b, int which, const uint32_t *ex
 
 fpa_result fpa_batch_set_operands(fpa_batch_t b, const uint32_t *a_exps, const uint32_t *a_limbs,
                                   const uint32_t *b_exps, const uint32_t *b_limbs) {
-  scatter_operand(b, 0, a_exps, a_limbs);
   int binary = 0;
   for (uint32_t s = 0; s < b->nseg; s++) binary |= b->sh[s].op != FPA_OP_SQRT;
-  if (binary) {
-    if (!b_exps || !b_limbs) return fpa_invalid_counts;
-    scatter_operand(b, 1, b_exps, b_limbs);
-  }
+  if (binary && (!b_exps || !b_limbs)) return fpa_invalid_counts;
+  scatter_operand(b, 0, a_exps, a_limbs);
+  if (binary) scatter_operand(b, 1, b_exps, b_limbs);
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
     size_t ne = (size_t)g->count * 4, nl = ne * (g->precision / 32);
@@ -219,14 +233,10 @@ fpa_result fpa_batch_run(fpa_batch_t b, void *stream) {
 }
 
 fpa_result fpa_batch_run_plain(fpa_batch_t b, void *stream) {
-  fpa_batch_segment *sd = (fpa_batch_segment *)malloc(sizeof(fpa_batch_segment) * (b->nseg ? b->nseg : 1));
-  FPA_HIP(hipMemcpy(sd, b->segs_dev, sizeof(fpa_batch_segment) * b->nseg, hipMemcpyDeviceToHost));
-  for (uint32_t s = 0; s < b->nseg; s++) {
-    int op = b->sh[s].op;
-    int rc = fpa_kernel_launch(op, &sd[s].args, fpa_kernel_get_variant(op, b->sh[s].precision), stream);
-    if (rc != 0) { fprintf(stderr, "plain launch of segment %u failed: %d\n", s, rc); exit(1); }
+  for (uint32_t k = 0; k < b->nseg; k++) {
+    int rc = launch_segment(&b->sh[b->order[k]], stream);
+    if (rc != 0) { fprintf(stderr, "plain launch of segment %u failed: %d\n", b->order[k], rc); exit(1); }
   }
-  free(sd);
   return fpa_success;
 }
 
@@ -252,23 +262,20 @@ fpa_result fpa_batch_destroy(fpa_batch_t b) {
   FPA_HIP(hipGraphExecDestroy(b->exec));
   FPA_HIP(hipGraphDestroy(b->graph));
   FPA_HIP(hipFree(b->pool));
-  FPA_HIP(hipFree(b->segs_dev));
-  FPA_HIP(hipFree(b->tiles_dev));
   FPA_HIP(hipHostFree(b->mirror));
   free(b->op_seg);
   free(b->op_slot);
   free(b->op_off);
+  free(b->order);
   free(b->sh);
   free(b);
   return fpa_success;
 }
 
 uint32_t fpa_batch_segments(fpa_batch_t b) { return b->nseg; }
-uint32_t fpa_batch_tiles(fpa_batch_t b) { return b->ntiles; }
-uint32_t fpa_batch_grid(fpa_batch_t b) { return b->grid; }
-size_t fpa_batch_device_bytes(fpa_batch_t b) {
-  return b->pool_bytes + sizeof(fpa_batch_segment) * b->nseg + 8 * (size_t)b->ntiles;
-}
+uint32_t fpa_batch_branches(fpa_batch_t b) { return b->branches; }
+uint64_t fpa_batch_blocks(fpa_batch_t b) { return b->blocks; }
+size_t fpa_batch_device_bytes(fpa_batch_t b) { return b->pool_bytes; }
 uint64_t fpa_batch_total_limbs(fpa_batch_t b) { return b->total_limbs; }
 
 /* ------------------------------ dot chains ------------------------------ */
@@ -310,7 +317,7 @@ fpa_result fpa_dot_plan_create(fpa_dot_plan_t *out, const uint32_t *lengths, uin
 
 fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, void *stream) {
   if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
-  if (fpa_impl_batch_kind(FPA_OP_MUL, x->precision) < 0) return fpa_invalid_precisions;
+  if (!fpa_impl_batch_supported(FPA_OP_MUL, x->precision)) return fpa_invalid_precisions;
   if (x->count != y->count || (uint64_t)x->count != p->total || r->count != p->count) return fpa_invalid_counts;
   fpa_dot_arguments d;
   memset(&d, 0, sizeof d);
@@ -342,7 +349,7 @@ fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, ui
   uint64_t total = 0;
   for (uint32_t j = 0; j < count; j++) total += lengths[j];
   if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
-  if (fpa_impl_batch_kind(FPA_OP_MUL, x->precision) < 0) return fpa_invalid_precisions;
+  if (!fpa_impl_batch_supported(FPA_OP_MUL, x->precision)) return fpa_invalid_precisions;
   if (x->count != y->count || (uint64_t)x->count != total || r->count != count) return fpa_invalid_counts;
   fpa_dot_plan_t p;
   fpa_dot_plan_create(&p, lengths, count);

@@ -6,11 +6,12 @@
  * the same per-element semantics and rounding.
  *
  * Batch: a list of element-wise ops, each with its own op (add/sub/mul/div/sqrt),
- * precision (256..16384, powers of two) and rounding mode.  fpa_batch_create groups the
- * ops into (op, precision, mode) segments, allocates the operand and result pools in
- * device memory once, builds a cost-sorted tile list, and captures one launch of the
- * persistent batch kernel in a hipGraph; fpa_batch_run replays that graph.  Host
- * operands and results are packed in op order: op i's exponent word at index i, its
+ * precision (any fpa_precision_supported value) and rounding mode.  fpa_batch_create
+ * groups the ops into (op, precision, mode) segments, allocates the operand and result
+ * pool in device memory once, and captures one hipGraph: a fork, `branches` independent
+ * chains of kernel nodes (one node per segment, segments dealt longest-first to the
+ * least-loaded branch) and a join.  fpa_batch_run replays that graph with one launch.
+ * Host operands and results are packed in op order: op i's exponent word at index i, its
  * limbs at offset sum_{j<i} precision_j/32.
  *
  * Dot chains: r[j] = x[o_j]*y[o_j] + ... + x[o_{j+1}-1]*y[o_{j+1}-1] where every
@@ -28,12 +29,8 @@
 extern "C" {
 #endif
 
-/* device-side segment descriptor read by the batch kernel */
-typedef struct {
-  fpa_arguments args;    /* count, precision, mode and r/a/b pool pointers of the segment */
-  uint32_t      kind;    /* compiled (op, precision) code path */
-  uint32_t      pad;
-} fpa_batch_segment;
+/* parallel branches of a batch graph made by fpa_batch_create (1..8 allowed) */
+#define FPA_BATCH_DEFAULT_BRANCHES 4
 
 /* device-side arguments of the dot-chain kernel */
 typedef struct {
@@ -49,34 +46,39 @@ typedef struct {
 
 typedef struct fpa_batch_s *fpa_batch_t;
 
-/* 1 if `precision` can appear in a batch (256, 512, ..., 16384) */
+/* 1 if `precision` can appear in a batch (same set as fpa_precision_supported) */
 int fpa_batch_precision_supported(uint32_t precision);
 
 /* Plan a batch of nops ops (op codes as fpa_op, modes as fpa_rounding_mode).
  * Returns fpa_invalid_precisions / fpa_invalid_counts / fpa_math_error (bad op). */
 fpa_result fpa_batch_create(fpa_batch_t *batch, uint32_t nops, const uint8_t *ops, const uint32_t *precisions,
                             const uint8_t *modes);
+/* Same with an explicit branch count (1..8; fpa_invalid_counts otherwise). */
+fpa_result fpa_batch_create_branches(fpa_batch_t *batch, uint32_t nops, const uint8_t *ops,
+                                     const uint32_t *precisions, const uint8_t *modes, uint32_t branches);
 /* Upload operands packed in op order (b may be NULL if no op is binary). */
 fpa_result fpa_batch_set_operands(fpa_batch_t batch, const uint32_t *a_exps, const uint32_t *a_limbs,
                                   const uint32_t *b_exps, const uint32_t *b_limbs);
-/* Replay the captured graph (one persistent kernel launch) on `stream` (hipStream_t). */
+/* Replay the captured graph (one graph launch) on `stream` (hipStream_t). */
 fpa_result fpa_batch_run(fpa_batch_t batch, void *stream);
-/* Same work as one plain per-segment kernel launch each (comparison path). */
+/* Same kernels launched one after another on `stream`, no graph (comparison path). */
 fpa_result fpa_batch_run_plain(fpa_batch_t batch, void *stream);
 /* Synchronise `stream` and copy the results back, packed in op order. */
 fpa_result fpa_batch_get_results(fpa_batch_t batch, void *stream, uint32_t *r_exps, uint32_t *r_limbs);
 fpa_result fpa_batch_destroy(fpa_batch_t batch);
 
-/* Plan facts: segments, tiles, persistent grid size (blocks), device bytes held. */
+/* Plan facts: segments (= kernel nodes), graph branches, workgroups over all nodes,
+ * device bytes held. */
 uint32_t fpa_batch_segments(fpa_batch_t batch);
-uint32_t fpa_batch_tiles(fpa_batch_t batch);
-uint32_t fpa_batch_grid(fpa_batch_t batch);
+uint32_t fpa_batch_branches(fpa_batch_t batch);
+uint64_t fpa_batch_blocks(fpa_batch_t batch);
 size_t   fpa_batch_device_bytes(fpa_batch_t batch);
 /* total limbs of the packed host arrays (sum of precision/32 over ops) */
 uint64_t fpa_batch_total_limbs(fpa_batch_t batch);
 
 /* Dot chains over device arrays: x and y hold all elements, r gets `count` results;
- * lengths[j] = elements in chain j (host array).  Same precision everywhere.
+ * lengths[j] = elements in chain j (host array).  Same precision everywhere, one of
+ * 256, 512, ..., 16384 (fpa_invalid_precisions otherwise).
  * Returns fpa_invalid_counts if x/y counts differ from sum(lengths) or r->count != count.
  * Synchronous (plans, runs on the fpa stream, waits, frees the plan). */
 fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, uint32_t count,

@@ -56,6 +56,25 @@ def test_precision_validation_without_device():
         assert F.lib.fpa_precision_supported(p) == 1
 
 
+def test_batch_validation_without_device():
+    u8 = ctypes.POINTER(ctypes.c_uint8)
+    h = ctypes.c_void_p()
+    def create(ops, precs, modes, branches=4):
+        o = np.array(ops, np.uint8)
+        p = np.array(precs, np.uint32)
+        m = np.array(modes, np.uint8)
+        return F.lib.fpa_batch_create_branches(ctypes.byref(h), len(o), o.ctypes.data_as(u8),
+                                               p.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               m.ctypes.data_as(u8), branches)
+    assert create([F.OPS["mul"]], [1536], [3]) == F.INVALID_PRECISIONS
+    assert create([F.OPS["neg"]], [256], [3]) == F.MATH_ERROR        # neg is not a batch op
+    assert create([F.OPS["add"]], [256], [5]) == F.MATH_ERROR        # no sixth rounding mode
+    assert create([F.OPS["add"]], [256], [3], branches=0) == F.INVALID_COUNTS
+    assert create([F.OPS["add"]], [256], [3], branches=9) == F.INVALID_COUNTS
+    for p in F.PRECISIONS:
+        assert F.lib.fpa_batch_precision_supported(p) == 1
+
+
 def test_variant_tables():
     for op in F.OPS:
         for p in F.PRECISIONS:

@@ -89,7 +89,27 @@ def test_batch_config3_distribution_matches_plain():
 
 def test_batch_rejects_bad_precision():
     with pytest.raises(ValueError):
-        F.Batch([F.OPS["mul"]], [3072])
+        F.Batch([F.OPS["mul"]], [1536])
+
+
+@pytest.mark.parametrize("branches", [1, 3, 8])
+def test_batch_branch_counts_agree(branches):
+    """Every branch count gives the same results as the plain launches; many small
+    segments (all five modes at every precision) so each branch gets several nodes."""
+    ops, precs, modes = W.mixed_ops(4000, 7, precisions=(256, 512, 1024, 2048, 3072, 4096),
+                                    modes=(0, 1, 2, 3, 4))
+    ae, al, be, bl = W.mixed_operands(ops, precs, 8)
+    B = F.Batch(ops, precs, modes, branches=branches)
+    say(f"[gpu] batch branches={branches} plan {B.info()}")
+    assert B.info()["branches"] == min(branches, B.info()["segments"])
+    B.set_operands(ae, al, be, bl)
+    B.run_plain()
+    pe, pl = B.results()
+    for rep in range(3):
+        B.run()
+        re, rl = B.results()
+        assert not compare_packed(precs, re, rl, pe, pl), f"replay {rep}"
+    B.close()
 
 
 @pytest.mark.parametrize("p", [256, 2048, 4096])
