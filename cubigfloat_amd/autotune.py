@@ -1,0 +1,137 @@
+"""Autotuner of the kernel shape per (op, precision).
+
+Each op at each precision has several compiled shapes (K limbs per lane, L lanes per
+value, K*L = precision/32).  For every (op, precision) this times every shape on the
+same synthetic operands (fpa_workload_generate kind 2: e ~ U[-1024, 1024], 1% zeros, 5%
+near-cancellation pairs), checks that every shape returns bit-identical results, and
+writes the fastest shape as one "op precision K L" line.  libfpa.so applies the file
+when it is loaded (tuning.txt beside the library, or $FPA_TUNING_FILE), so the fpa.h
+calls, the batch graphs, the server and the dot chains all use the tuned shapes.
+
+The reference picks one fixed shape per size in its precision switch
+(fpa_dispatch.h:294-321); this replaces that static choice.
+
+    python -m cubigfloat_amd.autotune --out cubigfloat_amd/tuning.txt
+"""
+import argparse
+import ctypes
+import sys
+import time
+
+import numpy as np
+
+from . import (FpaArguments, FpaArray, OPS, PRECISIONS, generate, get_variant, lib, set_variant,
+               variants)
+
+lib.fpa_kernel_time.argtypes = [ctypes.c_int, ctypes.POINTER(FpaArguments), ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+lib.fpa_kernel_time.restype = ctypes.c_int
+lib.fpa_kernel_tuning_applied.restype = ctypes.c_int
+
+TUNED_OPS = ("add", "sub", "mul", "div", "sqrt", "neg")
+
+
+def tune_count(op, precision, operand_bytes=256 << 20):
+    """Values per timed launch: one operand array of about operand_bytes, a quarter of
+    that for the long-running div/sqrt, at least 4096 and at most 2^22 values."""
+    per_value = 4 * (precision // 32 + 1)
+    c = operand_bytes // per_value
+    if op in ("div", "sqrt"):
+        c //= 4
+    return int(min(max(c, 4096), 1 << 22))
+
+
+def time_variant(op, precision, args, variant, target_ms=20.0):
+    ms = ctypes.c_float()
+    rc = lib.fpa_kernel_time(OPS[op], ctypes.byref(args), variant, 1, None, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"fpa_kernel_time({op}, {precision}, variant {variant}) returned {rc}")
+    reps = int(min(50, max(3, target_ms / max(ms.value, 1e-3))))
+    rc = lib.fpa_kernel_time(OPS[op], ctypes.byref(args), variant, reps, None, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"fpa_kernel_time({op}, {precision}, variant {variant}) returned {rc}")
+    return float(ms.value)
+
+
+def tune_one(op, precision, count=None, seed=1, log=None):
+    """Times every shape of (op, precision).  Returns (best (K, L), [(K, L, ms)], count)."""
+    count = count or tune_count(op, precision)
+    unary = op in ("sqrt", "neg")
+    ae, al, be, bl = generate(2, seed + precision, "sqrt" if op == "neg" else op, precision, count,
+                              binary=not unary)
+    A, R = FpaArray(count, precision), FpaArray(count, precision)
+    B = None
+    try:
+        A.upload(ae, al)
+        if not unary:
+            B = FpaArray(count, precision)
+            B.upload(be, bl)
+        args = FpaArguments()
+        args.count, args.precision, args.mode = count, precision, 3
+        args.r_exps, args.r_limbs = R.s.gpu_exps, R.s.gpu_limbs
+        args.a_exps, args.a_limbs = A.s.gpu_exps, A.s.gpu_limbs
+        src = B if B is not None else A
+        args.b_exps, args.b_limbs = src.s.gpu_exps, src.s.gpu_limbs
+        shapes = variants(op, precision)
+        timings, first = [], None
+        for v, (k, l) in enumerate(shapes):
+            ms = time_variant(op, precision, args, v)
+            e, lm = R.download()
+            if first is None:
+                first = (e, lm)
+            elif not (np.array_equal(e, first[0]) and np.array_equal(lm, first[1])):
+                raise RuntimeError(f"{op} {precision}: shape K={k} L={l} differs from K={shapes[0][0]} "
+                                   f"L={shapes[0][1]}")
+            timings.append((k, l, ms))
+        best = min(timings, key=lambda t: t[2])
+        if log:
+            parts = ", ".join(f"{k}x{l} {ms:.4g} ms{'*' if (k, l) == best[:2] else ''}" for k, l, ms in timings)
+            log(f"{op} {precision} ({count} values): {parts}")
+        return best[:2], timings, count
+    finally:
+        for x in (A, B, R):
+            if x is not None:
+                x.close()
+
+
+def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None):
+    """Returns {(op, precision): ((K, L), timings, count)}; also applies each choice."""
+    out = {}
+    for op in ops:
+        for p in precisions:
+            best, timings, count = tune_one(op, p, log=log)
+            out[(op, p)] = (best, timings, count)
+            set_variant(op, p, [s for s in variants(op, p)].index(best))
+    return out
+
+
+def write_table(path, table):
+    lines = ["# kernel shapes chosen by cubigfloat_amd.autotune, " + time.strftime("%Y-%m-%d"),
+             "# op precision K L   (per-shape mean launch time; * = chosen)"]
+    for (op, p), (best, timings, count) in sorted(table.items(), key=lambda kv: (OPS[kv[0][0]], kv[0][1])):
+        parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == best else "") for k, l, ms in timings)
+        lines.append(f"# {op} {p}, {count} values: {parts}")
+        lines.append(f"{OPS[op]} {p} {best[0]} {best[1]}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", required=True, help="tuning table to write")
+    ap.add_argument("--ops", default=",".join(TUNED_OPS))
+    ap.add_argument("--precisions", default=",".join(str(p) for p in PRECISIONS))
+    args = ap.parse_args()
+    ops = [o for o in args.ops.split(",") if o]
+    precs = [int(p) for p in args.precisions.split(",") if p]
+    for op in ops:
+        if op not in OPS:
+            sys.exit(f"unknown op {op}")
+    log = lambda s: print(s, flush=True)
+    table = tune(ops, precs, log=log)
+    write_table(args.out, table)
+    log(f"wrote {len(table)} entries to {args.out}")
+
+
+if __name__ == "__main__":
+    main()

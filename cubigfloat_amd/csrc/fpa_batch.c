@@ -28,8 +28,7 @@
     }                                                                                      \
   } while (0)
 
-int fpa_impl_batch_supported(int op, uint32_t precision);
-int fpa_impl_dot_launch(const fpa_dot_arguments *d, void *stream);
+int fpa_impl_dot_launch(const fpa_dot_arguments *d, int K, int L, void *stream);
 void *fpa_get_stream(void);
 
 #define FPA_BATCH_MAX_BRANCHES 8
@@ -317,7 +316,7 @@ fpa_result fpa_dot_plan_create(fpa_dot_plan_t *out, const uint32_t *lengths, uin
 
 fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, void *stream) {
   if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
-  if (!fpa_impl_batch_supported(FPA_OP_MUL, x->precision)) return fpa_invalid_precisions;
+  if (!fpa_precision_supported(x->precision)) return fpa_invalid_precisions;
   if (x->count != y->count || (uint64_t)x->count != p->total || r->count != p->count) return fpa_invalid_counts;
   fpa_dot_arguments d;
   memset(&d, 0, sizeof d);
@@ -332,7 +331,9 @@ fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rou
   d.y_limbs = y->gpu_limbs;
   d.offsets = p->offsets_dev;
   d.order = p->order_dev;
-  int rc = fpa_impl_dot_launch(&d, stream);
+  int K, L;
+  fpa_kernel_variant_shape(FPA_OP_MUL, x->precision, fpa_kernel_get_variant(FPA_OP_MUL, x->precision), &K, &L);
+  int rc = fpa_impl_dot_launch(&d, K, L, stream);
   if (rc != 0) { fprintf(stderr, "dot kernel launch failed: %d\n", rc); exit(1); }
   return fpa_success;
 }
@@ -349,7 +350,7 @@ fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, ui
   uint64_t total = 0;
   for (uint32_t j = 0; j < count; j++) total += lengths[j];
   if (x->precision != y->precision || r->precision != x->precision) return fpa_invalid_precisions;
-  if (!fpa_impl_batch_supported(FPA_OP_MUL, x->precision)) return fpa_invalid_precisions;
+  if (!fpa_precision_supported(x->precision)) return fpa_invalid_precisions;
   if (x->count != y->count || (uint64_t)x->count != total || r->count != count) return fpa_invalid_counts;
   fpa_dot_plan_t p;
   fpa_dot_plan_create(&p, lengths, count);

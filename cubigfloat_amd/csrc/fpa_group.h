@@ -50,56 +50,72 @@ __device__ __forceinline__ uint32_t and_all(const uint32_t (&x)[K]) {
   return r;
 }
 
-/* r = a + b + cin, returns the carry out of the chunk */
+/* A zero in a VGPR that the compiler cannot fold: carry chains with a zero operand then
+ * stay one v_addc_co / v_subb_co per limb instead of compare-and-select sequences. */
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z;
+  asm("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
+/* r = a + b + cin (cin 0 or 1), returns the carry out of the chunk */
 template <int K>
 __device__ __forceinline__ uint32_t chunk_add(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
                                               uint32_t cin) {
-  uint32_t c = cin;
+  unsigned int c = cin;
 #pragma unroll
   for (int i = 0; i < K; i++) {
-    uint64_t t = (uint64_t)a[i] + b[i] + c;
-    r[i] = (uint32_t)t;
-    c = (uint32_t)(t >> 32);
+    unsigned int o;
+    r[i] = __builtin_addc(a[i], b[i], c, &o);
+    c = o;
   }
   return c;
 }
 
-/* r = a - b - bin, returns the borrow out of the chunk */
+/* r = a - b - bin (bin 0 or 1), returns the borrow out of the chunk */
 template <int K>
 __device__ __forceinline__ uint32_t chunk_sub(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
                                               uint32_t bin) {
-  uint32_t c = bin;
+  unsigned int c = bin;
 #pragma unroll
   for (int i = 0; i < K; i++) {
-    uint64_t t = (uint64_t)a[i] - b[i] - c;
-    r[i] = (uint32_t)t;
-    c = (uint32_t)(t >> 63);
+    unsigned int o;
+    r[i] = __builtin_subc(a[i], b[i], c, &o);
+    c = o;
   }
   return c;
 }
 
-/* x += w at word 0, returns the carry out */
+/* x += w at word 0 (any 32-bit w), returns the carry out */
 template <int K>
 __device__ __forceinline__ uint32_t chunk_add_word(uint32_t (&x)[K], uint32_t w) {
-  uint32_t c = w;
+  unsigned int c, o;
+  x[0] = __builtin_addc(x[0], w, 0u, &o);
+  c = o;
+  if constexpr (K > 1) {
+    const uint32_t z = opaque_zero();
 #pragma unroll
-  for (int i = 0; i < K; i++) {
-    uint64_t t = (uint64_t)x[i] + c;
-    x[i] = (uint32_t)t;
-    c = (uint32_t)(t >> 32);
+    for (int i = 1; i < K; i++) {
+      x[i] = __builtin_addc(x[i], z, c, &o);
+      c = o;
+    }
   }
   return c;
 }
 
-/* x -= w at word 0, returns the borrow out */
+/* x -= w at word 0 (any 32-bit w), returns the borrow out */
 template <int K>
 __device__ __forceinline__ uint32_t chunk_sub_word(uint32_t (&x)[K], uint32_t w) {
-  uint32_t c = w;
+  unsigned int c, o;
+  x[0] = __builtin_subc(x[0], w, 0u, &o);
+  c = o;
+  if constexpr (K > 1) {
+    const uint32_t z = opaque_zero();
 #pragma unroll
-  for (int i = 0; i < K; i++) {
-    uint64_t t = (uint64_t)x[i] - c;
-    x[i] = (uint32_t)t;
-    c = (uint32_t)(t >> 63);
+    for (int i = 1; i < K; i++) {
+      x[i] = __builtin_subc(x[i], z, c, &o);
+      c = o;
+    }
   }
   return c;
 }
@@ -268,7 +284,8 @@ struct Group {
     uint32_t c = chunk_add_word(x, w);
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(x) == 0xFFFFFFFFu, 0u, cout);
-    chunk_add_word(x, ci);
+    /* a chunk passes a carry on only if it overflowed: rare, so test the whole wave */
+    if (__builtin_expect(__ballot(ci != 0) != 0, 0)) chunk_add_word(x, ci);
     return cout;
   }
 
@@ -278,7 +295,7 @@ struct Group {
     uint32_t c = chunk_sub_word(x, w);
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(x) == 0u, 0u, bout);
-    chunk_sub_word(x, bi);
+    if (__builtin_expect(__ballot(bi != 0) != 0, 0)) chunk_sub_word(x, bi);
     return bout;
   }
 

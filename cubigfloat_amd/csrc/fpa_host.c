@@ -7,6 +7,8 @@
  * size_t (the reference's uint32 products overflow at count*precision >= 2^32), the
  * device layout equals the host layout (no transpose), and each launch is checked.
  */
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -133,6 +135,49 @@ int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stre
   if (s < 0) return -1;
   if (args->count == 0) return 0;
   return launchers[op](args, shapes[s].K, shapes[s].L, stream);
+}
+
+int fpa_kernel_time(int op, const fpa_arguments *args, int variant, int reps, void *stream, float *ms) {
+  if (reps < 1 || !ms) return -1;
+  int rc = fpa_kernel_launch(op, args, variant, stream);
+  if (rc != 0) return rc;
+  hipEvent_t e0, e1;
+  FPA_HIP(hipEventCreate(&e0));
+  FPA_HIP(hipEventCreate(&e1));
+  FPA_HIP(hipEventRecord(e0, (hipStream_t)stream));
+  for (int i = 0; i < reps && rc == 0; i++) rc = fpa_kernel_launch(op, args, variant, stream);
+  FPA_HIP(hipEventRecord(e1, (hipStream_t)stream));
+  FPA_HIP(hipEventSynchronize(e1));
+  float t = 0.0f;
+  FPA_HIP(hipEventElapsedTime(&t, e0, e1));
+  FPA_HIP(hipEventDestroy(e0));
+  FPA_HIP(hipEventDestroy(e1));
+  *ms = t / (float)reps;
+  return rc;
+}
+
+/* tuning table applied when the library is loaded (host-only work: no HIP call) */
+static int tuning_applied;
+int fpa_kernel_tuning_applied(void) { return tuning_applied; }
+
+__attribute__((constructor)) static void load_default_tuning(void) {
+  const char *env = getenv("FPA_TUNING_FILE");
+  if (env && strcmp(env, "none") == 0) return;
+  if (env && *env) {
+    int n = fpa_kernel_load_tuning(env);
+    tuning_applied = n > 0 ? n : 0;
+    return;
+  }
+  Dl_info info;
+  if (!dladdr((void *)&load_default_tuning, &info) || !info.dli_fname) return;
+  char path[4096];
+  const char *slash = strrchr(info.dli_fname, '/');
+  size_t dir = slash ? (size_t)(slash - info.dli_fname) + 1 : 0;
+  if (dir + sizeof "tuning.txt" > sizeof path) return;
+  memcpy(path, info.dli_fname, dir);
+  memcpy(path + dir, "tuning.txt", sizeof "tuning.txt");
+  int n = fpa_kernel_load_tuning(path);
+  tuning_applied = n > 0 ? n : 0;
 }
 
 void fpa_set_stream(void *stream) { cur_stream = stream; }

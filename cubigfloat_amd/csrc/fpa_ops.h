@@ -475,23 +475,31 @@ __device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g,
 }
 
 /* Multiply-subtract step shared by div and sqrt:
- *   (rtop : R) -= q * (ttop : T), with R, T chunks of n words and rtop signed. */
+ *   (rtop : R) -= q * (ttop : T), with R, T chunks of n words and rtop signed.
+ * rtop is kept by the group's top lane only (other lanes hold don't-care values): the
+ * word a lane's chunk pushes out, `out`, is the top lane's own in that lane. */
 template <int K, int L>
 __device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], int64_t &rtop, uint32_t q,
                                           const uint32_t (&T)[K], uint32_t ttop) {
-  uint32_t c = 0, bw = 0;
+  uint32_t c = 0;
+  unsigned int bw = 0;
 #pragma unroll
   for (int i = 0; i < K; i++) {
     uint64_t p = (uint64_t)q * T[i] + c;
     c = (uint32_t)(p >> 32);
-    uint64_t d = (uint64_t)R[i] - (uint32_t)p - bw;
-    R[i] = (uint32_t)d;
-    bw = (uint32_t)(d >> 63);
+    unsigned int o;
+    R[i] = __builtin_subc(R[i], (uint32_t)p, bw, &o);
+    bw = o;
   }
   uint32_t out = c + bw;                     /* <= 2^32-1: high product word is <= 2^32-2 */
   uint32_t bout = g.sub_lane_words(R, g.from_prev(out));
-  uint32_t out_top = g.bcast(out, L - 1);
-  rtop = rtop - (int64_t)out_top - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
+  rtop = rtop - (int64_t)out - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
+}
+
+/* true in every lane of the group when the top lane's rtop is negative */
+template <int L>
+__device__ __forceinline__ bool top_negative(const Group<L> &g, int64_t rtop) {
+  return g.any(g.gl == L - 1 && rtop < 0);
 }
 
 /* -------------------------------- div -------------------------------- */
@@ -499,6 +507,7 @@ __device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], i
 template <int K, int L>
 __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                          uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
+  static_assert(K >= 2, "the top lane must hold the two leading words after a shift");
   constexpr int n = K * L;
   uint32_t act = k_div_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
   if (act != FPA_A_DIV) {
@@ -522,21 +531,27 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
   const uint32_t v = fpa_recip_3by2(d1, d0);
 
   for (int j = 0; j <= n; j++) {
-    /* R <- R * 2^32 */
-    int64_t rtop = g.template word<K, n - 1>(R);
+    /* R <- R * 2^32; the word leaving the top becomes rtop (top lane only) */
+    int64_t rtop = R[K - 1];
     uint32_t in = g.from_prev(R[K - 1]);
 #pragma unroll
     for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
     R[0] = in;
-    const uint32_t u2 = (uint32_t)rtop, u1 = g.template word<K, n - 1>(R), u0 = g.template word<K, n - 2>(R);
+    /* the top lane holds the three leading words (rtop, R[K-1], R[K-2]) and estimates
+     * the digit; the other lanes' estimates are discarded by the broadcast */
+    const uint32_t u2 = (uint32_t)rtop, u1 = R[K - 1], u0 = R[K - 2];
     uint32_t q;
     if (u2 > d1 || (u2 == d1 && u1 >= d0)) q = 0xFFFFFFFFu;
     else q = fpa_div3by2(u2, u1, u0, d1, d0, v);
+    q = g.bcast(q, L - 1);
     msub_step<K, L>(g, R, rtop, q, B, 0u);
-    for (int guard = 0; rtop < 0 && guard < 4; guard++) {
-      uint32_t c = g.add(R, R, B);
-      rtop += c;
-      q -= 1u;
+    if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
+      /* add back (the estimate exceeded the digit): rare, off the common path */
+      for (int guard = 0; top_negative(g, rtop) && guard < 4; guard++) {
+        uint32_t c = g.add(R, R, B);
+        rtop += c;
+        q -= 1u;
+      }
     }
     if (g.gl == 0) lds[n - j] = q;
   }
@@ -581,6 +596,7 @@ __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g,
 template <int K, int L>
 __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                           uint32_t &re, uint32_t (&Sr)[K]) {
+  static_assert(K >= 2, "the top lane must hold the two leading words after a shift");
   constexpr int n = K * L;
   uint32_t act = k_sqrt_tab[fpa_classify(ea)];
   if (act != FPA_A_SQRT) {
@@ -631,15 +647,16 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
   }
   uint32_t v = 0;
   for (int pos = n - 2; pos >= 0; pos--) {
-    /* R <- R * 2^32 */
-    uint32_t rt = g.template word<K, n - 1>(R);
+    /* R <- R * 2^32; rtop (top lane only) takes the word leaving the top */
+    uint32_t rt = R[K - 1];
     uint32_t in = g.from_prev(R[K - 1]);
 #pragma unroll
     for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
     R[0] = in;
     rtop = (rtop << 32) | rt;
-    /* estimate: floor((R/2) / S) from the top words, S = T/2 */
-    const uint32_t r1 = g.template word<K, n - 1>(R), r2 = g.template word<K, n - 2>(R);
+    /* estimate: floor((R/2) / S) from the top words, S = T/2; the top lane holds them
+     * and its digit is broadcast */
+    const uint32_t r1 = R[K - 1], r2 = R[K - 2];
     const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
     const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
     const uint32_t u0 = (r1 << 31) | (r2 >> 1);
@@ -649,26 +666,30 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     uint32_t d;
     if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
     else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
+    d = g.bcast(d, L - 1);
     /* R -= d * (T + d*2^(32 pos)) */
     if (g.gl == 0) Tl[pos] = d;
     wsync();
     uint32_t Tr[K];
     lds_load(Tr, Tl + g.gl * K);
     msub_step<K, L>(g, R, rtop, d, Tr, dtop);
-    for (int guard = 0; rtop < 0 && guard < 6; guard++) {
-      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
-      uint32_t c = g.add(R, R, Tr);
-      rtop += (int64_t)c + dtop;
-      d -= 1u;
-      uint32_t Y[K];
+    if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
+      /* add back (the estimate exceeded the digit): rare, off the common path */
+      for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
+        /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
+        uint32_t c = g.add(R, R, Tr);
+        rtop += (int64_t)c + dtop;
+        d -= 1u;
+        uint32_t Y[K];
 #pragma unroll
-      for (int i = 0; i < K; i++) {
-        bool here = g.gl * K + i == pos;
-        Y[i] = here ? d : 0u;
-        Tr[i] = here ? d : Tr[i];
+        for (int i = 0; i < K; i++) {
+          bool here = g.gl * K + i == pos;
+          Y[i] = here ? d : 0u;
+          Tr[i] = here ? d : Tr[i];
+        }
+        c = g.add(R, R, Y);
+        rtop += (int64_t)c;
       }
-      c = g.add(R, R, Y);
-      rtop += (int64_t)c;
     }
     wsync();
     /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */
@@ -682,6 +703,8 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
   /* root S (n words), remainder (rtop : R) = X - S^2 */
   uint32_t S[K], D[K];
   lds_load(S, Sl + g.gl * K);
+  rtop = (int64_t)(((uint64_t)g.bcast((uint32_t)((uint64_t)rtop >> 32), L - 1) << 32) |
+                   g.bcast((uint32_t)rtop, L - 1));
   const uint32_t rem_nz = (rtop != 0 || g.any(or_all(R) != 0)) ? 1u : 0u;
   uint32_t gt;
   if (rtop > 0) gt = 1;

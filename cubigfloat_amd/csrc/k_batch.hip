@@ -4,9 +4,9 @@
  * fpa_dot_kernel: one group of L lanes per dot product, accumulator kept in registers;
  * acc_{i+1} = RN(acc_i + RN(x_i * y_i)) in index order.  Groups take dot products in
  * the host-given order (longest first) so the groups sharing a wave have similar lengths.
+ * Instantiated for every compiled shape; the host picks the dispatcher's mul shape.
  */
 #include "fpa_kernel_common.h"
-#include "fpa_batch_shapes.h"
 #include "../../include/fpa_batch.h"
 
 namespace fpa {
@@ -51,24 +51,16 @@ __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
 
 }  // namespace fpa
 
-extern "C" int fpa_impl_batch_supported(int op, uint32_t precision) {
-#define FPA_KIND_FIND(op_, n, K, L) \
-  if (op == op_ && precision == 32u * n) return 1;
-  FPA_BATCH_SHAPES(FPA_KIND_FIND)
-#undef FPA_KIND_FIND
-  return 0;
-}
-
-extern "C" int fpa_impl_dot_launch(const fpa_dot_arguments *d, void *stream) {
+extern "C" int fpa_impl_dot_launch(const fpa_dot_arguments *d, int K, int L, void *stream) {
   if (d->count == 0) return 0;
-#define FPA_DOT_CASE(op_, n, K, L)                                                               \
-  if (op_ == 3 && d->precision == 32u * n) {                                                     \
-    uint64_t waves = ((uint64_t)d->count + (64 / L) - 1) / (64 / L);                             \
-    hipLaunchKernelGGL((fpa::fpa_dot_kernel<K, L>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, \
+#define FPA_DOT_CASE(n, K_, L_)                                                                  \
+  if (d->precision == 32u * n && K == K_ && L == L_) {                                           \
+    uint64_t waves = ((uint64_t)d->count + (64 / L_) - 1) / (64 / L_);                           \
+    hipLaunchKernelGGL((fpa::fpa_dot_kernel<K_, L_>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, \
                        (hipStream_t)stream, *d);                                                 \
     return (int)hipGetLastError();                                                               \
   }
-  FPA_BATCH_SHAPES(FPA_DOT_CASE)
+  FPA_SHAPES(FPA_DOT_CASE)
 #undef FPA_DOT_CASE
   return -1;
 }

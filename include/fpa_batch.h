@@ -77,8 +77,9 @@ size_t   fpa_batch_device_bytes(fpa_batch_t batch);
 uint64_t fpa_batch_total_limbs(fpa_batch_t batch);
 
 /* Dot chains over device arrays: x and y hold all elements, r gets `count` results;
- * lengths[j] = elements in chain j (host array).  Same precision everywhere, one of
- * 256, 512, ..., 16384 (fpa_invalid_precisions otherwise).
+ * lengths[j] = elements in chain j (host array).  Same precision everywhere, any
+ * fpa_precision_supported value (fpa_invalid_precisions otherwise); the kernel shape is
+ * the dispatcher's current mul shape for that precision.
  * Returns fpa_invalid_counts if x/y counts differ from sum(lengths) or r->count != count.
  * Synchronous (plans, runs on the fpa stream, waits, frees the plan). */
 fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, uint32_t count,

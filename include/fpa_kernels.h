@@ -60,8 +60,19 @@ int fpa_kernel_get_variant(int op, uint32_t precision);
 int fpa_kernel_set_variant(int op, uint32_t precision, int variant);
 
 /* Load "op precision K L" lines (as written by the autotuner) into the dispatch table.
- * Returns the number of entries applied, or -1 if the file cannot be read. */
+ * Returns the number of entries applied, or -1 if the file cannot be read.
+ * When the library is loaded it applies, if present, the file named by the environment
+ * variable FPA_TUNING_FILE, else tuning.txt in the directory holding the library
+ * (FPA_TUNING_FILE=none skips both). */
 int fpa_kernel_load_tuning(const char *path);
+
+/* Number of entries the load-time tuning file applied (0 if none was read). */
+int fpa_kernel_tuning_applied(void);
+
+/* Autotuner timing (no counterpart in the reference): launches variant `variant` of
+ * op once untimed, then `reps` times between two HIP events on `stream`, waits, and
+ * stores the mean kernel time in ms in *ms.  Returns 0, or the launch / HIP error. */
+int fpa_kernel_time(int op, const fpa_arguments *args, int variant, int reps, void *stream, float *ms);
 
 /* Stream used by the fpa.h functions (a hipStream_t; NULL = null stream). */
 void fpa_set_stream(void *stream);

@@ -81,21 +81,38 @@ def test_variant_tables():
             vs = F.variants(op, p)
             assert vs, (op, p)
             for k, l in vs:
-                assert k * l == p // 32 and 64 % l == 0
-            assert F.get_variant(op, p) == 0
+                assert k * l == p // 32 and 64 % l == 0 and l <= 32
+            assert len(set(vs)) == len(vs)
+            assert 0 <= F.get_variant(op, p) < len(vs)
+    keep = F.get_variant("mul", 4096)
     assert F.set_variant("mul", 4096, 99) == -1
     assert F.set_variant("mul", 4096, 1) == 0 and F.get_variant("mul", 4096) == 1
-    F.set_variant("mul", 4096, 0)
+    F.set_variant("mul", 4096, keep)
 
 
 def test_tuning_file(tmp_path):
     f = tmp_path / "tune.txt"
-    k, l = F.variants("mul", 4096)[1]
+    keep = F.get_variant("mul", 4096)
+    k, l = F.variants("mul", 4096)[2]
     f.write_text(f"# op precision K L\n3 4096 {k} {l}\n3 4096 3 3\n")
     assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == 1
-    assert F.get_variant("mul", 4096) == 1
-    F.set_variant("mul", 4096, 0)
+    assert F.get_variant("mul", 4096) == 2
+    F.set_variant("mul", 4096, keep)
     assert F.lib.fpa_kernel_load_tuning(b"/nonexistent/file") == -1
+
+
+def test_load_time_tuning_matches_committed_table():
+    """The library applies tuning.txt beside it when loaded; every line of the committed
+    table names a compiled shape, so every line is applied."""
+    path = os.path.join(ROOT, "cubigfloat_amd", "tuning.txt")
+    if not os.path.exists(path):
+        pytest.skip("no tuning table committed")
+    lines = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    if os.environ.get("FPA_TUNING_FILE") is None:
+        assert F.lib.fpa_kernel_tuning_applied() == len(lines)
+    for op, prec, k, l in lines:
+        name = {v: n for n, v in F.OPS.items()}[int(op)]
+        assert (int(k), int(l)) in F.variants(name, int(prec))
 
 
 def test_scalar_helpers_vs_python():
@@ -144,3 +161,25 @@ def test_workload_generator_properties():
     both = (ae > 8) & (be > 8)
     close = both & (np.abs((ae.astype(np.int64) >> 1) - (be.astype(np.int64) >> 1)) <= 1) & ((ae ^ be) & 1 == 1)
     assert 0.03 < close.mean() < 0.08
+
+
+def test_autotune_table_round_trip(tmp_path):
+    """A table written by the autotuner is read back by the library's loader."""
+    from cubigfloat_amd import autotune as T
+    keep = {(op, p): F.get_variant(op, p) for op in ("mul", "div") for p in (1024, 4096)}
+    table = {}
+    for (op, p) in keep:
+        vs = F.variants(op, p)
+        pick = vs[-1]
+        table[(op, p)] = (pick, [(k, l, 1.0 + i) for i, (k, l) in enumerate(vs)], 1234)
+    f = tmp_path / "t.txt"
+    T.write_table(str(f), table)
+    try:
+        assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == len(table)
+        for (op, p) in keep:
+            assert F.get_variant(op, p) == len(F.variants(op, p)) - 1
+    finally:
+        for (op, p), v in keep.items():
+            F.set_variant(op, p, v)
+    assert T.tune_count("add", 256) == 1 << 22
+    assert T.tune_count("div", 16384) == (256 << 20) // (4 * 513) // 4
