@@ -20,6 +20,8 @@ ROUND_TRUNCATE, ROUND_UP, ROUND_DOWN, ROUND_TIES_TO_EVEN, ROUND_TIES_AWAY = rang
 MODES = {"rz": 0, "ru": 1, "rd": 2, "rne": 3, "rna": 4}
 SUCCESS, MATH_ERROR, INVALID_PRECISIONS, INVALID_COUNTS = range(4)
 OPS = {"neg": 0, "add": 1, "sub": 2, "mul": 3, "div": 4, "sqrt": 5}
+# variant-table ids: the element-wise ops plus the dot-chain kernel (FPA_KERNEL_DOT)
+KERNELS = dict(OPS, dot=6)
 PRECISIONS = (256, 512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 16384)
 BIAS = 0x3FFFFFF3
 
@@ -162,19 +164,19 @@ def apply(op, r, a, b=None, mode=ROUND_TIES_TO_EVEN):
 def variants(op, precision):
     """[(K, L), ...] compiled kernel shapes for (op, precision)."""
     out = []
-    for v in range(lib.fpa_kernel_variants(OPS[op], precision)):
+    for v in range(lib.fpa_kernel_variants(KERNELS[op], precision)):
         k, l = ctypes.c_int(), ctypes.c_int()
-        lib.fpa_kernel_variant_shape(OPS[op], precision, v, ctypes.byref(k), ctypes.byref(l))
+        lib.fpa_kernel_variant_shape(KERNELS[op], precision, v, ctypes.byref(k), ctypes.byref(l))
         out.append((k.value, l.value))
     return out
 
 
 def set_variant(op, precision, variant):
-    return lib.fpa_kernel_set_variant(OPS[op], precision, variant)
+    return lib.fpa_kernel_set_variant(KERNELS[op], precision, variant)
 
 
 def get_variant(op, precision):
-    return lib.fpa_kernel_get_variant(OPS[op], precision)
+    return lib.fpa_kernel_get_variant(KERNELS[op], precision)
 
 
 def generate(kind, seed, op, precision, count, first=0, binary=True):
@@ -237,6 +239,7 @@ lib.fpa_dot_chains.argtypes = [_P, _P, _P, _u32p, ctypes.c_uint32, ctypes.c_int]
 lib.fpa_dot_plan_create.argtypes = [ctypes.POINTER(_vp), _u32p, ctypes.c_uint32]
 lib.fpa_dot_plan_run.argtypes = [_vp, _P, _P, _P, ctypes.c_int, _vp]
 lib.fpa_dot_plan_destroy.argtypes = [_vp]
+lib.fpa_dot_plan_time.argtypes = [_vp, _P, _P, _P, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_float)]
 
 BATCH_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192, 16384)
 

@@ -4,7 +4,8 @@ Each op at each precision has several compiled shapes (K limbs per lane, L lanes
 value, K*L = precision/32).  For every (op, precision) this times every shape on the
 same synthetic operands (fpa_workload_generate kind 2: e ~ U[-1024, 1024], 1% zeros, 5%
 near-cancellation pairs), checks that every shape returns bit-identical results, and
-writes the fastest shape as one "op precision K L" line.  libfpa.so applies the file
+writes the fastest shape as one "op precision K L" line (op 6 = the dot-chain kernel,
+timed on chains of U[8, 512] elements).  libfpa.so applies the file
 when it is loaded (tuning.txt beside the library, or $FPA_TUNING_FILE), so the fpa.h
 calls, the batch graphs, the server and the dot chains all use the tuned shapes.
 
@@ -20,15 +21,15 @@ import time
 
 import numpy as np
 
-from . import (FpaArguments, FpaArray, OPS, PRECISIONS, generate, get_variant, lib, set_variant,
-               variants)
+from . import (DotPlan, FpaArguments, FpaArray, KERNELS, OPS, PRECISIONS, generate, get_variant, lib,
+               set_variant, variants)
 
 lib.fpa_kernel_time.argtypes = [ctypes.c_int, ctypes.POINTER(FpaArguments), ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
 lib.fpa_kernel_time.restype = ctypes.c_int
 lib.fpa_kernel_tuning_applied.restype = ctypes.c_int
 
-TUNED_OPS = ("add", "sub", "mul", "div", "sqrt", "neg")
+TUNED_OPS = ("add", "sub", "mul", "div", "sqrt", "neg", "dot")
 
 
 def tune_count(op, precision, operand_bytes=256 << 20):
@@ -53,8 +54,57 @@ def time_variant(op, precision, args, variant, target_ms=20.0):
     return float(ms.value)
 
 
+def tune_dot(precision, count=None, seed=1, log=None):
+    """Dot chains (fpa_dot_plan_run): chains of U[8, 512] elements, `count` elements in
+    all (an eighth of tune_count's mul figure by default)."""
+    count = count or max(tune_count("mul", precision) // 8, 4096)
+    rng = np.random.default_rng(seed + precision)
+    lengths = []
+    while sum(lengths) < count:
+        lengths.append(int(rng.integers(8, 513)))
+    lengths[-1] -= sum(lengths) - count
+    if lengths[-1] <= 0:
+        lengths.pop()
+    lengths = np.array(lengths, np.uint32)
+    total = int(lengths.sum())
+    xe, xl, ye, yl = generate(1, seed + precision, "mul", precision, total)
+    X, Y, R = FpaArray(total, precision), FpaArray(total, precision), FpaArray(len(lengths), precision)
+    plan = DotPlan(lengths)
+    keep = get_variant("dot", precision)
+    try:
+        X.upload(xe, xl)
+        Y.upload(ye, yl)
+        shapes = variants("dot", precision)
+        timings, first = [], None
+        for v, (k, l) in enumerate(shapes):
+            set_variant("dot", precision, v)
+            ms = ctypes.c_float()
+            rc = lib.fpa_dot_plan_time(plan.h, R.ref, X.ref, Y.ref, 3, 3, None, ctypes.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"fpa_dot_plan_time({precision}, variant {v}) returned {rc}")
+            e, lm = R.download()
+            if first is None:
+                first = (e, lm)
+            elif not (np.array_equal(e, first[0]) and np.array_equal(lm, first[1])):
+                raise RuntimeError(f"dot {precision}: shape K={k} L={l} differs from K={shapes[0][0]} "
+                                   f"L={shapes[0][1]}")
+            timings.append((k, l, float(ms.value)))
+        best = min(timings, key=lambda t: t[2])
+        if log:
+            parts = ", ".join(f"{k}x{l} {ms:.4g} ms{'*' if (k, l) == best[:2] else ''}" for k, l, ms in timings)
+            log(f"dot {precision} ({total} elements, {len(lengths)} chains): {parts}")
+        return best[:2], timings, total
+    finally:
+        set_variant("dot", precision, keep)
+        for x in (X, Y, R):
+            x.close()
+        plan.close()
+
+
 def tune_one(op, precision, count=None, seed=1, log=None):
     """Times every shape of (op, precision).  Returns (best (K, L), [(K, L, ms)], count)."""
+    if op == "dot":
+        return tune_dot(precision, count, seed, log)
     count = count or tune_count(op, precision)
     unary = op in ("sqrt", "neg")
     ae, al, be, bl = generate(2, seed + precision, "sqrt" if op == "neg" else op, precision, count,
@@ -108,10 +158,10 @@ def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None):
 def write_table(path, table):
     lines = ["# kernel shapes chosen by cubigfloat_amd.autotune, " + time.strftime("%Y-%m-%d"),
              "# op precision K L   (per-shape mean launch time; * = chosen)"]
-    for (op, p), (best, timings, count) in sorted(table.items(), key=lambda kv: (OPS[kv[0][0]], kv[0][1])):
+    for (op, p), (best, timings, count) in sorted(table.items(), key=lambda kv: (KERNELS[kv[0][0]], kv[0][1])):
         parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == best else "") for k, l, ms in timings)
-        lines.append(f"# {op} {p}, {count} values: {parts}")
-        lines.append(f"{OPS[op]} {p} {best[0]} {best[1]}")
+        lines.append(f"# {op} {p}, {count} {'elements' if op == 'dot' else 'values'}: {parts}")
+        lines.append(f"{KERNELS[op]} {p} {best[0]} {best[1]}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -125,7 +175,7 @@ def main():
     ops = [o for o in args.ops.split(",") if o]
     precs = [int(p) for p in args.precisions.split(",") if p]
     for op in ops:
-        if op not in OPS:
+        if op not in KERNELS:
             sys.exit(f"unknown op {op}")
     log = lambda s: print(s, flush=True)
     table = tune(ops, precs, log=log)

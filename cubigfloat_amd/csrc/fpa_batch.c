@@ -332,10 +332,30 @@ fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rou
   d.offsets = p->offsets_dev;
   d.order = p->order_dev;
   int K, L;
-  fpa_kernel_variant_shape(FPA_OP_MUL, x->precision, fpa_kernel_get_variant(FPA_OP_MUL, x->precision), &K, &L);
+  fpa_kernel_variant_shape(FPA_KERNEL_DOT, x->precision, fpa_kernel_get_variant(FPA_KERNEL_DOT, x->precision), &K, &L);
   int rc = fpa_impl_dot_launch(&d, K, L, stream);
   if (rc != 0) { fprintf(stderr, "dot kernel launch failed: %d\n", rc); exit(1); }
   return fpa_success;
+}
+
+int fpa_dot_plan_time(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, int reps, void *stream,
+                      float *ms) {
+  if (reps < 1 || !ms) return -1;
+  fpa_result rc = fpa_dot_plan_run(p, r, x, y, mode, stream);
+  if (rc != fpa_success) return (int)rc;
+  hipEvent_t e0, e1;
+  FPA_HIP(hipEventCreate(&e0));
+  FPA_HIP(hipEventCreate(&e1));
+  FPA_HIP(hipEventRecord(e0, (hipStream_t)stream));
+  for (int i = 0; i < reps; i++) fpa_dot_plan_run(p, r, x, y, mode, stream);
+  FPA_HIP(hipEventRecord(e1, (hipStream_t)stream));
+  FPA_HIP(hipEventSynchronize(e1));
+  float t = 0.0f;
+  FPA_HIP(hipEventElapsedTime(&t, e0, e1));
+  FPA_HIP(hipEventDestroy(e0));
+  FPA_HIP(hipEventDestroy(e1));
+  *ms = t / (float)reps;
+  return 0;
 }
 
 fpa_result fpa_dot_plan_destroy(fpa_dot_plan_t p) {

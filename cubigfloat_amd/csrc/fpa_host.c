@@ -56,7 +56,7 @@ static const int prec_n[] = { 8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 512 };
 #define NPREC ((int)(sizeof(prec_n) / sizeof(prec_n[0])))
 
 /* dispatch table: variant index per (op, precision); 0 = first compiled shape */
-static int variant_tab[FPA_OP_COUNT][NPREC];
+static int variant_tab[FPA_OP_COUNT + 1][NPREC];   /* + FPA_KERNEL_DOT */
 static void *cur_stream = NULL;
 
 static int prec_index(uint32_t precision) {
@@ -68,11 +68,15 @@ static int prec_index(uint32_t precision) {
 
 int fpa_precision_supported(uint32_t precision) { return prec_index(precision) >= 0; }
 
-/* variant v of limb count n -> index into shapes[], or -1 */
-static int shape_of(int n, int v) {
+/* shapes an op may use: every compiled one, except that the dot kernel exists only for
+ * K <= FPA_DOT_MAX_K */
+static int shape_allowed(int op, int s) { return op != FPA_KERNEL_DOT || shapes[s].K <= FPA_DOT_MAX_K; }
+
+/* variant v of (op, limb count n) -> index into shapes[], or -1 */
+static int shape_of(int op, int n, int v) {
   int k = 0;
   for (int i = 0; i < NSHAPES; i++) {
-    if (shapes[i].n != n) continue;
+    if (shapes[i].n != n || !shape_allowed(op, i)) continue;
     if (k == v) return i;
     k++;
   }
@@ -80,15 +84,15 @@ static int shape_of(int n, int v) {
 }
 
 int fpa_kernel_variants(int op, uint32_t precision) {
-  if (op < 0 || op >= FPA_OP_COUNT || prec_index(precision) < 0) return 0;
+  if (op < 0 || op > FPA_KERNEL_DOT || prec_index(precision) < 0) return 0;
   int n = (int)(precision / 32), k = 0;
-  for (int i = 0; i < NSHAPES; i++) k += shapes[i].n == n;
+  for (int i = 0; i < NSHAPES; i++) k += shapes[i].n == n && shape_allowed(op, i);
   return k;
 }
 
 int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *K, int *L) {
   if (fpa_kernel_variants(op, precision) <= variant || variant < 0) return -1;
-  int s = shape_of((int)(precision / 32), variant);
+  int s = shape_of(op, (int)(precision / 32), variant);
   if (K) *K = shapes[s].K;
   if (L) *L = shapes[s].L;
   return 0;
@@ -96,13 +100,13 @@ int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *K, in
 
 int fpa_kernel_get_variant(int op, uint32_t precision) {
   int p = prec_index(precision);
-  if (op < 0 || op >= FPA_OP_COUNT || p < 0) return -1;
+  if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
   return variant_tab[op][p];
 }
 
 int fpa_kernel_set_variant(int op, uint32_t precision, int variant) {
   int p = prec_index(precision);
-  if (op < 0 || op >= FPA_OP_COUNT || p < 0) return -1;
+  if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
   if (variant < 0 || variant >= fpa_kernel_variants(op, precision)) return -1;
   variant_tab[op][p] = variant;
   return 0;
@@ -131,7 +135,7 @@ int fpa_kernel_load_tuning(const char *path) {
 
 int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stream) {
   if (op < 0 || op >= FPA_OP_COUNT || !args) return -1;
-  int s = prec_index(args->precision) < 0 ? -1 : shape_of((int)(args->precision / 32), variant);
+  int s = prec_index(args->precision) < 0 ? -1 : shape_of(op, (int)(args->precision / 32), variant);
   if (s < 0) return -1;
   if (args->count == 0) return 0;
   return launchers[op](args, shapes[s].K, shapes[s].L, stream);

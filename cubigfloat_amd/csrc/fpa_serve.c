@@ -5,8 +5,9 @@
  * from a first-fit free list (256-byte granules, neighbours coalesced on free).
  * A request's region holds, per (op, mode) group, its results first and then its
  * operands, so one host->device copy uploads all operands and one device->host copy
- * returns all results.  Each slot has a stream, an event and a pinned staging buffer.
- * One scheduler thread owns the slots and the pool; the queue is shared with submitters.
+ * returns all results.  Each slot has a stream, a pinned staging buffer and a worker
+ * thread; one scheduler thread admits queued requests into free slots.  The queue, the
+ * pool and the slot states are guarded by one mutex.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
@@ -236,13 +237,15 @@ static void pool_free(pool_t *p, size_t off, size_t size) {
 
 /* -------------------------------- server -------------------------------- */
 typedef struct {
+  struct fpa_server_s *S;
   hipStream_t st;
-  hipEvent_t ev;
   uint8_t *pin;
-  int busy;
+  int busy;                /* reserved by the scheduler until the worker has finished */
   fpa_request *req;
   layout_t L;
   size_t off, bytes;
+  pthread_t thr;
+  pthread_cond_t cv;       /* scheduler -> worker: a request was assigned */
 } slot_t;
 
 struct fpa_server_s {
@@ -256,43 +259,52 @@ struct fpa_server_s {
   pthread_cond_t cv_work, cv_done;
   fpa_request **q;
   uint32_t qhead, qcount, qcap;
-  int stop;
+  int stop, stop_workers;
   double t0;
   uint64_t budget_waits;
   fpa_request *waited_head;      /* head request already counted as waiting for memory */
 };
 
-static void finish(struct fpa_server_s *S, fpa_request *r, int status) {
+/* One worker per slot: stages the request's operands, runs it on the slot's stream,
+ * waits for it and returns its results, so a large request's host copies never hold
+ * up the admission or completion of others. */
+static void *worker(void *arg) {
+  slot_t *sl = (slot_t *)arg;
+  struct fpa_server_s *S = sl->S;
   pthread_mutex_lock(&S->mu);
-  r->status = status;
-  pthread_cond_broadcast(&S->cv_done);
+  for (;;) {
+    while (!sl->req && !S->stop_workers) pthread_cond_wait(&sl->cv, &S->mu);
+    if (!sl->req) break;
+    fpa_request *r = sl->req;
+    pthread_mutex_unlock(&S->mu);
+    scatter_inputs(r, &sl->L, sl->pin);
+    uint8_t *dev = S->dev + sl->off;
+    FPA_HIP(hipMemcpyAsync(dev + sl->L.r_bytes, sl->pin + sl->L.r_bytes, sl->L.in_bytes, hipMemcpyHostToDevice,
+                           sl->st));
+    launch_groups(r, &sl->L, dev, sl->st);
+    FPA_HIP(hipMemcpyAsync(sl->pin, dev, sl->L.r_bytes, hipMemcpyDeviceToHost, sl->st));
+    FPA_HIP(hipStreamSynchronize(sl->st));
+    gather_results(r, &sl->L, sl->pin);
+    pthread_mutex_lock(&S->mu);
+    r->t_done = mono_s() - S->t0;
+    pool_free(&S->pool, sl->off, sl->bytes);
+    sl->req = NULL;
+    sl->busy = 0;
+    r->status = fpa_success;
+    pthread_cond_broadcast(&S->cv_done);
+    pthread_cond_signal(&S->cv_work);
+  }
   pthread_mutex_unlock(&S->mu);
+  return NULL;
 }
 
+/* Admission in arrival order: the queue head starts as soon as a slot is free and the
+ * pool can hold it; a head that does not fit waits (later requests do not pass it). */
 static void *scheduler(void *arg) {
   struct fpa_server_s *S = (struct fpa_server_s *)arg;
+  pthread_mutex_lock(&S->mu);
   for (;;) {
-    int progress = 0, busy = 0;
-    /* completions */
-    for (uint32_t i = 0; i < S->nslots; i++) {
-      slot_t *sl = &S->slots[i];
-      if (!sl->busy) continue;
-      hipError_t e = hipEventQuery(sl->ev);
-      if (e == hipErrorNotReady) { busy++; continue; }
-      FPA_HIP(e);
-      gather_results(sl->req, &sl->L, sl->pin);
-      sl->req->t_done = mono_s() - S->t0;
-      pthread_mutex_lock(&S->mu);
-      pool_free(&S->pool, sl->off, sl->bytes);
-      pthread_mutex_unlock(&S->mu);
-      sl->busy = 0;
-      finish(S, sl->req, fpa_success);
-      progress = 1;
-    }
-    /* admissions, in arrival order */
-    pthread_mutex_lock(&S->mu);
-    for (;;) {
-      if (S->qcount == 0) break;
+    while (S->qcount > 0) {
       uint32_t fs;
       for (fs = 0; fs < S->nslots && S->slots[fs].busy; fs++) {}
       if (fs == S->nslots) break;
@@ -306,7 +318,6 @@ static void *scheduler(void *arg) {
         S->qcount--;
         r->status = rc;
         pthread_cond_broadcast(&S->cv_done);
-        progress = 1;
         continue;
       }
       size_t off;
@@ -316,45 +327,18 @@ static void *scheduler(void *arg) {
       }
       S->qhead = (S->qhead + 1) % S->qcap;
       S->qcount--;
-      pthread_mutex_unlock(&S->mu);
       sl->busy = 1;
       sl->req = r;
       sl->off = off;
       sl->bytes = bytes;
       r->slot = fs;
       r->t_admit = mono_s() - S->t0;
-      scatter_inputs(r, &sl->L, sl->pin);
-      uint8_t *dev = S->dev + off;
-      FPA_HIP(hipMemcpyAsync(dev + sl->L.r_bytes, sl->pin + sl->L.r_bytes, sl->L.in_bytes, hipMemcpyHostToDevice,
-                             sl->st));
-      launch_groups(r, &sl->L, dev, sl->st);
-      FPA_HIP(hipMemcpyAsync(sl->pin, dev, sl->L.r_bytes, hipMemcpyDeviceToHost, sl->st));
-      FPA_HIP(hipEventRecord(sl->ev, sl->st));
-      progress = 1;
-      busy++;
-      pthread_mutex_lock(&S->mu);
+      pthread_cond_signal(&sl->cv);
     }
-    if (S->stop && S->qcount == 0 && busy == 0) {
-      pthread_mutex_unlock(&S->mu);
-      break;
-    }
-    if (!progress) {
-      if (busy == 0) {
-        /* idle: sleep until a submission (or stop) */
-        struct timespec ts;
-        clock_gettime(CLOCK_REALTIME, &ts);
-        ts.tv_nsec += 2000000;
-        if (ts.tv_nsec >= 1000000000) { ts.tv_sec++; ts.tv_nsec -= 1000000000; }
-        if (S->qcount == 0 && !S->stop) pthread_cond_timedwait(&S->cv_work, &S->mu, &ts);
-        pthread_mutex_unlock(&S->mu);
-      } else {
-        pthread_mutex_unlock(&S->mu);
-        usleep(10);
-      }
-    } else {
-      pthread_mutex_unlock(&S->mu);
-    }
+    if (S->stop && S->qcount == 0) break;
+    pthread_cond_wait(&S->cv_work, &S->mu);
   }
+  pthread_mutex_unlock(&S->mu);
   return NULL;
 }
 
@@ -368,16 +352,19 @@ fpa_result fpa_server_create(fpa_server_t *out, size_t budget_bytes, uint32_t sl
   pool_init(&S->pool, S->budget);
   S->nslots = slots;
   S->slots = (slot_t *)calloc(slots, sizeof(slot_t));
-  for (uint32_t i = 0; i < slots; i++) {
-    FPA_HIP(hipStreamCreateWithFlags(&S->slots[i].st, hipStreamNonBlocking));
-    FPA_HIP(hipEventCreateWithFlags(&S->slots[i].ev, hipEventDisableTiming));
-    FPA_HIP(hipHostMalloc((void **)&S->slots[i].pin, S->max_req, hipHostMallocDefault));
-  }
   S->qcap = 1024;
   S->q = (fpa_request **)malloc(sizeof(fpa_request *) * S->qcap);
   pthread_mutex_init(&S->mu, NULL);
   pthread_cond_init(&S->cv_work, NULL);
   pthread_cond_init(&S->cv_done, NULL);
+  for (uint32_t i = 0; i < slots; i++) {
+    slot_t *sl = &S->slots[i];
+    sl->S = S;
+    FPA_HIP(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+    FPA_HIP(hipHostMalloc((void **)&sl->pin, S->max_req, hipHostMallocDefault));
+    pthread_cond_init(&sl->cv, NULL);
+    pthread_create(&sl->thr, NULL, worker, sl);
+  }
   S->t0 = mono_s();
   pthread_create(&S->thr, NULL, scheduler, S);
   *out = S;
@@ -391,10 +378,16 @@ fpa_result fpa_server_destroy(fpa_server_t S) {
   pthread_cond_broadcast(&S->cv_work);
   pthread_mutex_unlock(&S->mu);
   pthread_join(S->thr, NULL);
+  /* the queue is empty; let the workers finish what they hold, then stop them */
+  pthread_mutex_lock(&S->mu);
+  S->stop_workers = 1;
+  for (uint32_t i = 0; i < S->nslots; i++) pthread_cond_signal(&S->slots[i].cv);
+  pthread_mutex_unlock(&S->mu);
+  for (uint32_t i = 0; i < S->nslots; i++) pthread_join(S->slots[i].thr, NULL);
   for (uint32_t i = 0; i < S->nslots; i++) {
     FPA_HIP(hipStreamDestroy(S->slots[i].st));
-    FPA_HIP(hipEventDestroy(S->slots[i].ev));
     FPA_HIP(hipHostFree(S->slots[i].pin));
+    pthread_cond_destroy(&S->slots[i].cv);
     layout_free(&S->slots[i].L);
   }
   FPA_HIP(hipFree(S->dev));

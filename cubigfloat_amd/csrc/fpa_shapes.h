@@ -20,4 +20,9 @@
   X(256, 16, 16) X(256, 32, 8)  X(256, 8, 32)                \
   X(512, 16, 32) X(512, 32, 16)
 
+/* The dot-chain kernel (one mul core and one add core per element) is compiled only for
+ * shapes with K <= FPA_DOT_MAX_K: above that its add core no longer inlines and becomes a
+ * real call, which no kernel of this library may contain. */
+#define FPA_DOT_MAX_K 20
+
 #endif

@@ -4,7 +4,8 @@
  * fpa_dot_kernel: one group of L lanes per dot product, accumulator kept in registers;
  * acc_{i+1} = RN(acc_i + RN(x_i * y_i)) in index order.  Groups take dot products in
  * the host-given order (longest first) so the groups sharing a wave have similar lengths.
- * Instantiated for every compiled shape; the host picks the dispatcher's mul shape.
+ * Instantiated for the compiled shapes with K <= FPA_DOT_MAX_K; the host picks the
+ * dispatcher's FPA_KERNEL_DOT shape.
  */
 #include "fpa_kernel_common.h"
 #include "../../include/fpa_batch.h"
@@ -54,6 +55,7 @@ __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
 extern "C" int fpa_impl_dot_launch(const fpa_dot_arguments *d, int K, int L, void *stream) {
   if (d->count == 0) return 0;
 #define FPA_DOT_CASE(n, K_, L_)                                                                  \
+  if constexpr (K_ <= FPA_DOT_MAX_K)                                                             \
   if (d->precision == 32u * n && K == K_ && L == L_) {                                           \
     uint64_t waves = ((uint64_t)d->count + (64 / L_) - 1) / (64 / L_);                           \
     hipLaunchKernelGGL((fpa::fpa_dot_kernel<K_, L_>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, \

@@ -79,7 +79,7 @@ uint64_t fpa_batch_total_limbs(fpa_batch_t batch);
 /* Dot chains over device arrays: x and y hold all elements, r gets `count` results;
  * lengths[j] = elements in chain j (host array).  Same precision everywhere, any
  * fpa_precision_supported value (fpa_invalid_precisions otherwise); the kernel shape is
- * the dispatcher's current mul shape for that precision.
+ * the dispatcher's FPA_KERNEL_DOT variant for that precision.
  * Returns fpa_invalid_counts if x/y counts differ from sum(lengths) or r->count != count.
  * Synchronous (plans, runs on the fpa stream, waits, frees the plan). */
 fpa_result fpa_dot_chains(fpa_t r, fpa_t x, fpa_t y, const uint32_t *lengths, uint32_t count,
@@ -90,6 +90,10 @@ typedef struct fpa_dot_plan_s *fpa_dot_plan_t;
 fpa_result fpa_dot_plan_create(fpa_dot_plan_t *plan, const uint32_t *lengths, uint32_t count);
 fpa_result fpa_dot_plan_run(fpa_dot_plan_t plan, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, void *stream);
 fpa_result fpa_dot_plan_destroy(fpa_dot_plan_t plan);
+/* Autotuner timing: one untimed run, then `reps` runs between two HIP events on
+ * `stream`; mean ms per run in *ms.  Returns 0, or an fpa_result / -1. */
+int fpa_dot_plan_time(fpa_dot_plan_t plan, fpa_t r, fpa_t x, fpa_t y, fpa_rounding_mode mode, int reps,
+                      void *stream, float *ms);
 
 #ifdef __cplusplus
 }

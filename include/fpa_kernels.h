@@ -42,13 +42,19 @@ typedef enum {
   FPA_OP_COUNT = 6
 } fpa_op;
 
+/* Variant-table id of the dot-chain kernel (fpa_batch.h): accepted by
+ * fpa_kernel_variants / _variant_shape / _get_variant / _set_variant and in tuning
+ * files, not by fpa_kernel_launch. */
+#define FPA_KERNEL_DOT 6
+
 /* Launch op over args->count elements with kernel variant `variant` (index into
  * fpa_kernel_variants) on `stream` (a hipStream_t, NULL = null stream).
  * Returns 0 on success, otherwise a hipError_t value (or -1 for an unknown
  * op/precision/variant).  count == 0 launches nothing. */
 int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stream);
 
-/* Number of compiled variants for (op, precision); 0 if the precision is unsupported. */
+/* Number of compiled variants for (op, precision); 0 if the precision is unsupported.
+ * op may also be FPA_KERNEL_DOT (same shapes as mul). */
 int fpa_kernel_variants(int op, uint32_t precision);
 
 /* Shape of a variant: limbs per lane, lanes per value. Returns 0, or -1 if invalid. */

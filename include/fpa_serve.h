@@ -9,9 +9,10 @@
  * The server owns one device pool of `budget_bytes` (allocated once, kept resident) and
  * `slots` HIP streams.  A scheduler thread admits queued requests in arrival order as
  * soon as a slot is free and the pool can hold the request's operands and results;
- * admitted requests upload, run (one kernel per (op, mode) group, via the tuned
- * dispatcher) and download on their own stream, so requests overlap on the device and
- * new ones start while others are still running.  Results are identical to running each
+ * each slot's worker thread then stages, uploads, runs (one kernel per (op, mode)
+ * group, via the tuned dispatcher) and downloads its request on the slot's own stream,
+ * so requests overlap on the device and on the host, and new ones start while others
+ * are still running.  Results are identical to running each
  * request alone (every element is computed independently).
  */
 #ifndef FPA_SERVE_H

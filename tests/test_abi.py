@@ -76,7 +76,7 @@ def test_batch_validation_without_device():
 
 
 def test_variant_tables():
-    for op in F.OPS:
+    for op in F.KERNELS:
         for p in F.PRECISIONS:
             vs = F.variants(op, p)
             assert vs, (op, p)
@@ -111,7 +111,7 @@ def test_load_time_tuning_matches_committed_table():
     if os.environ.get("FPA_TUNING_FILE") is None:
         assert F.lib.fpa_kernel_tuning_applied() == len(lines)
     for op, prec, k, l in lines:
-        name = {v: n for n, v in F.OPS.items()}[int(op)]
+        name = {v: n for n, v in F.KERNELS.items()}[int(op)]
         assert (int(k), int(l)) in F.variants(name, int(prec))
 
 
@@ -166,7 +166,7 @@ def test_workload_generator_properties():
 def test_autotune_table_round_trip(tmp_path):
     """A table written by the autotuner is read back by the library's loader."""
     from cubigfloat_amd import autotune as T
-    keep = {(op, p): F.get_variant(op, p) for op in ("mul", "div") for p in (1024, 4096)}
+    keep = {(op, p): F.get_variant(op, p) for op in ("mul", "div", "dot") for p in (1024, 4096)}
     table = {}
     for (op, p) in keep:
         vs = F.variants(op, p)

@@ -1,0 +1,58 @@
+"""Properties of the gfx950 machine code in libfpa.so, read from the library's own
+offload bundles (objcopy + clang-offload-bundler + llvm-objdump of this build).
+
+Every kernel body must be fully inlined: no kernel may call a device function.  Kernels
+whose cores were compiled as real calls (pointer arguments into the private segment,
+reached through flat loads/stores) are the only ones that ever failed on the device,
+so the build keeps them out: the dot-chain kernel is instantiated only for shapes whose
+body inlines, and this test fails if any call appears."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "cubigfloat_amd", "libfpa.so")
+LLVM = "/opt/rocm/llvm/bin"
+MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+
+
+def code_objects(tmp_path):
+    fat = tmp_path / "fatbin.bin"
+    subprocess.run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin", LIB, str(fat)], check=True)
+    data = fat.read_bytes()
+    starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
+    assert starts, "no offload bundle in libfpa.so"
+    out = []
+    for i, s in enumerate(starts):
+        chunk = data[s:starts[i + 1] if i + 1 < len(starts) else len(data)]
+        src = tmp_path / f"bundle{i}.bin"
+        dst = tmp_path / f"bundle{i}.co"
+        src.write_bytes(chunk)
+        subprocess.run([f"{LLVM}/clang-offload-bundler", "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
+                        f"--input={src}", f"--output={dst}", "--unbundle"], check=True)
+        out.append(dst)
+    return out
+
+
+@pytest.mark.skipif(not (os.path.exists(LIB) and shutil.which("objcopy") and os.path.exists(f"{LLVM}/llvm-objdump")),
+                    reason="library or LLVM tools missing")
+def test_no_kernel_calls_a_function(tmp_path):
+    calls = {}
+    kernels = 0
+    for co in code_objects(tmp_path):
+        dis = subprocess.run([f"{LLVM}/llvm-objdump", "-d", str(co)], check=True, capture_output=True,
+                             text=True).stdout
+        cur = None
+        for line in dis.splitlines():
+            m = re.match(r"^[0-9a-f]+ <(\S+)>:", line)
+            if m:
+                cur = m.group(1)
+                kernels += "fpa_kernel" in cur or "fpa_dot_kernel" in cur
+                continue
+            if "s_swappc_b64" in line:
+                calls[cur] = calls.get(cur, 0) + 1
+    assert kernels > 0
+    assert not calls, f"kernels with device-function calls: {sorted(calls)[:8]}"

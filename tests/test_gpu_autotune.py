@@ -8,7 +8,7 @@ from cubigfloat_amd import autotune as T
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("op,p", [("mul", 1024), ("div", 2048), ("sqrt", 4096), ("add", 8192)])
+@pytest.mark.parametrize("op,p", [("mul", 1024), ("div", 2048), ("sqrt", 4096), ("add", 8192), ("dot", 2048)])
 def test_tune_one_small(op, p):
     print(f"[gpu] autotune {op} {p}", flush=True)
     keep = F.get_variant(op, p)

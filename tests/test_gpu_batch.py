@@ -127,6 +127,24 @@ def test_dot_chains_vs_oracle(p):
         assert eq.all(), f"{int((~eq).sum())} mismatches"
 
 
+@pytest.mark.parametrize("p", [512, 2048, 8192])
+def test_dot_every_shape_vs_oracle(p):
+    rng = np.random.default_rng(p + 1)
+    lengths = rng.integers(0, 24, 70).astype(np.uint32)
+    total = int(lengths.sum())
+    xe, xl, ye, yl = F.generate(2, 5, "mul", p, total)
+    oe, ol = O.dot(p, lengths, xe, xl, ye, yl, mode="rne")
+    keep = F.get_variant("dot", p)
+    try:
+        for v, (k, l) in enumerate(F.variants("dot", p)):
+            F.set_variant("dot", p, v)
+            say(f"[gpu] dot p={p} shape K={k} L={l}")
+            re, rl = F.dot_chains(p, lengths, xe, xl, ye, yl)
+            assert O.results_equal(re, rl, oe, ol, p).all(), (k, l)
+    finally:
+        F.set_variant("dot", p, keep)
+
+
 def test_dot_config4_shape_sample():
     lengths, xe, xl, ye, yl = W.dot_workload(256, 2048, 11)
     say(f"[gpu] dot config-4 sample chains=256 elements={int(lengths.sum())}")
