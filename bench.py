@@ -25,8 +25,12 @@ sys.path.insert(0, ROOT)
 
 # MI355X: 256 CUs, 4 SIMD16 per CU, 2.4 GHz peak engine clock, 8 TB/s HBM3E.
 CUS, CLOCK_HZ, HBM_GBS = 256, 2.4e9, 8000.0
-# 32x32->64 multiplies (v_mad_u64_u32) per second at quarter rate: 256 CU * 64 lanes/4 clk * 2.4 GHz
-MAD_PEAK_T = CUS * 16 * CLOCK_HZ / 1e12
+# INT32 VALU issue ceiling: one 32x32->64 multiply-add (v_mad_u64_u32) per lane per clock,
+# 256 CU * 4 SIMD * 16 lanes * 2.4 GHz.  (A quarter-rate multiplier would give a quarter of
+# this; the probe below measures the device's sustained rate.)
+MAD_PEAK_T = CUS * 64 * CLOCK_HZ / 1e12
+MAD_QUARTER_RATE_T = MAD_PEAK_T / 4
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096.md")
 
 
 class Hip:
@@ -64,7 +68,7 @@ def algorithmic(op, p):
     read and written."""
     n = p // 32
     prods = {"mul": n * n, "div": n * n, "sqrt": n * n // 2}.get(op, 0)
-    nops = 1 if op == "sqrt" else 2
+    nops = 1 if op in ("sqrt", "neg") else 2
     byts = (nops + 1) * (4 * n + 4)
     return prods, byts
 
@@ -131,6 +135,31 @@ def cpu_baseline(op, p, sample_in, gpu_out, threads):
         if x is not None:
             x.close()
     return cnt * reps / t, reps * cnt, bool(ok.all()), int((~ok).sum())
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by the same counters on a copy kernel of known size):
+    lines 'traffic_bytes_per_launch <op><precision> <bytes>'.  {} if absent."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
+                out[f[1]] = float(f[2])
+    return out
+
+
+def kernels_only(F, hip, args):
+    """Short run for hardware-counter passes: neg (a pure copy of known bytes), mul and add
+    at 4096 bits over the same 2^20-value workload, one warmup and `steps` launches each."""
+    res = []
+    for op in ("neg", "mul", "add"):
+        print(f"[bench] kernels-only {op} 4096 x{args.count}", file=sys.stderr, flush=True)
+        r, _, _ = time_op(F, hip, op, 4096, args.count, args.steps, 1, 2, 2, 16)
+        r["algorithmic_bytes_per_launch"] = algorithmic(op, 4096)[1] * args.count
+        res.append(r)
+    print(json.dumps({"kernels_only": res}), flush=True)
 
 
 def pct(x, q):
@@ -286,6 +315,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and MPFR checks")
     ap.add_argument("--only", default="", help="comma list of extra ops to time (default all)")
     ap.add_argument("--configs", default="3,4,5", help="batched/chained/served configurations to run ('' for none)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only time neg/mul/add at 4096 bits (for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -295,6 +326,9 @@ def main():
 
     import cubigfloat_amd as F
     hip = Hip()
+    if args.kernels_only:
+        kernels_only(F, hip, args)
+        return
     F.lib.fpa_ubench_mad.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]
     blocks, iters = 256 * 4 * 8, 4096
     ms = ctypes.c_float()
@@ -350,6 +384,7 @@ def main():
         configs["config5"] = bench_config5(F, 400, 4 << 30)
 
     ops = head["ops_per_s"]
+    traffic = measured_traffic().get("mul4096")
     res = {
         "metric": "4096-bit correctly-rounded mul/s", "value": ops, "unit": "mul/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_launch"],
@@ -360,10 +395,15 @@ def main():
                    "kernel_shape": F.variants("mul", p)[F.get_variant("mul", p)]},
         "roofline": {"bound": "valu", "achieved": head["tproducts_per_s"], "peak": MAD_PEAK_T,
                      "unit": "T 32x32->64 products/s", "frac": head["tproducts_per_s"] / MAD_PEAK_T,
-                     "traffic": None, "peak_measured_mad_probe": mad_measured_t,
+                     "traffic": traffic,
+                     "traffic_algorithmic": algorithmic("mul", p)[1] * count,
+                     "peak_measured_mad_probe": mad_measured_t,
                      "frac_of_measured": head["tproducts_per_s"] / mad_measured_t,
-                     "note": "INT32 VALU multiply roofline (no MFMA, no HBM bound at 4096 bits); "
-                             "peak = v_mad_u64_u32 at quarter rate, 256 CU x 16 lanes/clk x 2.4 GHz"},
+                     "peak_if_quarter_rate": MAD_QUARTER_RATE_T,
+                     "note": "INT32 VALU multiply roofline (no MFMA; at 4096 bits mul is far from the HBM "
+                             "bound); achieved = n^2 = 16384 word products per mul x mul/s; peak = one "
+                             "v_mad_u64_u32 per lane per clock, 256 CU x 64 lanes x 2.4 GHz; traffic = "
+                             "HBM bytes per launch from rocprofv3 FETCH_SIZE+WRITE_SIZE (profiles/)"},
         "cpu_baseline": cpu,
         "checks_bit_exact_vs_mpfr": checks,
         "extras": extras,
