@@ -77,9 +77,11 @@ def time_op(F, hip, op, p, count, steps, warmup, kind, seed, check_sample):
     ae, al, be, bl = F.generate(kind, seed, op, p, count, binary=op != "sqrt")
     A, R = F.FpaArray(count, p), F.FpaArray(count, p)
     B = F.FpaArray(count, p) if op != "sqrt" else None
+    t_up = time.perf_counter()
     A.upload(ae, al)
     if B is not None:
         B.upload(be, bl)
+    t_up = time.perf_counter() - t_up
     for _ in range(warmup):
         assert F.apply(op, R, A, B) == F.SUCCESS
     hip.sync()
@@ -99,7 +101,7 @@ def time_op(F, hip, op, p, count, steps, warmup, kind, seed, check_sample):
     n = p // 32
     sample = slice(0, check_sample)
     res = {"op": op, "precision": p, "count": count, "ms_per_launch": float(np.mean(per)),
-           "ms_min": float(np.min(per)), "wall_s": wall}
+           "ms_min": float(np.min(per)), "wall_s": wall, "upload_s": t_up}
     res["ops_per_s"] = count / (res["ms_per_launch"] / 1e3)
     prods, byts = algorithmic(op, p)
     res["tproducts_per_s"] = prods * res["ops_per_s"] / 1e12
@@ -409,6 +411,11 @@ def main():
         "extras": extras,
         "configs": configs,
         "wall_s": head["wall_s"],
+        "host_buffer_rate": {"upload_s": head["upload_s"],
+                             "mul_per_s_including_operand_upload":
+                                 count / (head["upload_s"] + head["ms_per_launch"] / 1e3),
+                             "note": "fpa_copy_array_to_device of both operand arrays from pageable host "
+                                     "memory, then one launch; not the metric"},
     }
     print(json.dumps(res), flush=True)
 
