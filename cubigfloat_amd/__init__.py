@@ -152,6 +152,68 @@ def neg(r, a):
     return lib.fpa_neg(r.ref, a.ref)
 
 
+lib.fpa_set.argtypes = [_P, _P, ctypes.c_int]
+lib.fpa_sgn.argtypes = [ctypes.c_void_p, _P]
+lib.fpa_cmp.argtypes = [ctypes.c_void_p, _P, _P]
+
+
+class DeviceInt32:
+    """A device int32 array (hipMalloc) for fpa_sgn / fpa_cmp results."""
+
+    _hip = None
+
+    def __init__(self, count):
+        if DeviceInt32._hip is None:
+            h = ctypes.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            DeviceInt32._hip = h
+        self.count = count
+        self.ptr = ctypes.c_void_p()
+        assert DeviceInt32._hip.hipMalloc(ctypes.byref(self.ptr), max(4 * count, 4)) == 0
+
+    def download(self):
+        out = np.empty(self.count, np.int32)
+        if self.count:
+            assert DeviceInt32._hip.hipMemcpy(out.ctypes.data, self.ptr, 4 * self.count, 2) == 0  # device to host
+        return out
+
+    def close(self):
+        if self.ptr:
+            DeviceInt32._hip.hipFree(self.ptr)
+            self.ptr = None
+
+
+def set_precision(r, x, mode=ROUND_TIES_TO_EVEN):
+    """fpa_set: r (its own precision) = x rounded in `mode`."""
+    return lib.fpa_set(r.ref, x.ref, mode)
+
+
+def sgn(x):
+    """fpa_sgn into a device array; returns the int32 results on the host."""
+    d = DeviceInt32(x.count)
+    try:
+        rc = lib.fpa_sgn(d.ptr, x.ref)
+        if rc != SUCCESS:
+            raise RuntimeError(f"fpa_sgn returned {rc}")
+        return d.download()
+    finally:
+        d.close()
+
+
+def cmp(a, b):
+    """fpa_cmp into a device array; returns the int32 results on the host."""
+    d = DeviceInt32(a.count)
+    try:
+        rc = lib.fpa_cmp(d.ptr, a.ref, b.ref)
+        if rc != SUCCESS:
+            raise RuntimeError(f"fpa_cmp returned {rc}")
+        return d.download()
+    finally:
+        d.close()
+
+
 def apply(op, r, a, b=None, mode=ROUND_TIES_TO_EVEN):
     """Run op by name ('add', 'mul', ...) through the fpa.h entry point."""
     if op in ("sqrt",):

@@ -38,6 +38,8 @@ int fpa_impl_launch_sub(const fpa_arguments *a, int K, int L, void *stream);
 int fpa_impl_launch_mul(const fpa_arguments *a, int K, int L, void *stream);
 int fpa_impl_launch_div(const fpa_arguments *a, int K, int L, void *stream);
 int fpa_impl_launch_sqrt(const fpa_arguments *a, int K, int L, void *stream);
+int fpa_impl_launch_sgn(uint32_t count, const uint32_t *x_exps, int32_t *r, void *stream);
+int fpa_impl_launch_setcmp(int op, const fpa_arguments *args, int K, int L, void *stream);
 
 typedef int (*launch_fn)(const fpa_arguments *, int, int, void *);
 static const launch_fn launchers[FPA_OP_COUNT] = {
@@ -335,18 +337,56 @@ fpa_result fpa_set_double(fpa_t r, double *doubles) {
   return upload_staged(r, se, sl);
 }
 
-/* Not on the element-wise arithmetic path; not provided by this build. */
+/* set / sgn / cmp (fpa_dispatch.h:88-153): the kernels in k_setcmp.hip, launched on the
+ * fpa stream with the dispatcher's neg shape (a copy-like kernel) for the precision. */
+static void copy_shape(uint32_t precision, int *K, int *L) {
+  fpa_kernel_variant_shape(FPA_OP_NEG, precision, fpa_kernel_get_variant(FPA_OP_NEG, precision), K, L);
+}
+
 fpa_result fpa_set(fpa_t r, fpa_t x, fpa_rounding_mode mode) {
-  (void)r; (void)x; (void)mode;
-  return fpa_math_error;
+  if (r->count != x->count) return fpa_invalid_counts;
+  if (!fpa_precision_supported(r->precision) || !fpa_precision_supported(x->precision))
+    return fpa_invalid_precisions;
+  if ((int)mode < 0 || (int)mode > 4) return fpa_math_error;
+  fpa_arguments a;
+  memset(&a, 0, sizeof a);
+  a.count = r->count;
+  a.precision = r->precision;
+  a.set_source_precision = x->precision;
+  a.mode = mode;
+  a.r_exps = r->gpu_exps;
+  a.r_limbs = r->gpu_limbs;
+  a.a_exps = x->gpu_exps;
+  a.a_limbs = x->gpu_limbs;
+  int K, L;
+  copy_shape(r->precision, &K, &L);
+  int rc = fpa_impl_launch_setcmp(1, &a, K, L, cur_stream);
+  if (rc != 0) { fprintf(stderr, "fpa_set: launch failed (%d)\n", rc); exit(1); }
+  return fpa_success;
 }
+
 fpa_result fpa_sgn(int32_t *r, fpa_t x) {
-  (void)r; (void)x;
-  return fpa_math_error;
+  int rc = fpa_impl_launch_sgn(x->count, x->gpu_exps, r, cur_stream);
+  if (rc != 0) { fprintf(stderr, "fpa_sgn: launch failed (%d)\n", rc); exit(1); }
+  return fpa_success;
 }
+
 fpa_result fpa_cmp(int32_t *r, fpa_t a, fpa_t b) {
-  (void)r;
   if (a->count != b->count) return fpa_invalid_counts;
   if (a->precision != b->precision) return fpa_invalid_precisions;
-  return fpa_math_error;
+  if (!fpa_precision_supported(a->precision)) return fpa_invalid_precisions;
+  fpa_arguments g;
+  memset(&g, 0, sizeof g);
+  g.count = a->count;
+  g.precision = a->precision;
+  g.r_exps = (uint32_t *)r;
+  g.a_exps = a->gpu_exps;
+  g.a_limbs = a->gpu_limbs;
+  g.b_exps = b->gpu_exps;
+  g.b_limbs = b->gpu_limbs;
+  int K, L;
+  copy_shape(a->precision, &K, &L);
+  int rc = fpa_impl_launch_setcmp(0, &g, K, L, cur_stream);
+  if (rc != 0) { fprintf(stderr, "fpa_cmp: launch failed (%d)\n", rc); exit(1); }
+  return fpa_success;
 }

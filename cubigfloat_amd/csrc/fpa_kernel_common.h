@@ -10,6 +10,14 @@
 #include "fpa_ops.h"
 #include "fpa_shapes.h"
 
+/* Optional occupancy target (waves per SIMD) for every element-wise kernel, set at build
+ * time (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n); unset = the compiler's own choice. */
+#ifdef FPA_WAVES_PER_EU
+#define FPA_KERNEL_ATTRS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FPA_WAVES_PER_EU)))
+#else
+#define FPA_KERNEL_ATTRS __launch_bounds__(256)
+#endif
+
 namespace fpa {
 
 /* LDS words of scratch per value for each op (neg, add, sub, mul, div, sqrt) */
@@ -26,7 +34,7 @@ struct ScratchWords {
 };
 
 template <int K, int L, int OP>
-__global__ __launch_bounds__(256) void fpa_kernel(fpa_arguments a) {
+__global__ FPA_KERNEL_ATTRS void fpa_kernel(fpa_arguments a) {
   constexpr int VPW = 64 / L;
   constexpr int SW = ScratchWords<OP, K, L>::value;
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];

@@ -74,9 +74,15 @@ fpa_result fpa_set_ui(fpa_t r, uint64_t *ui);
 fpa_result fpa_set_si(fpa_t r, int64_t *si);
 fpa_result fpa_set_float(fpa_t r, float *floats);
 fpa_result fpa_set_double(fpa_t r, double *doubles);
+/* r = x rounded to r's precision in `mode` (any two supported precisions; special
+ * classes copied; E > 0x7FFFFFFF after rounding up -> INF).  Same results as mpfr_set,
+ * the reference harness's gold (tests.c:580-605); the reference's own kernel is
+ * unfinished (fpa_warp_set.h:134-199).  fpa_invalid_counts if the counts differ. */
 fpa_result fpa_set(fpa_t r, fpa_t x, fpa_rounding_mode mode);
 
-/* r[i] = sign / three-way compare into a device int32 array -- fpa_dispatch.h:109-153 */
+/* r[i] = sign / three-way compare (-1, 0, +1) into a device int32 array, by the
+ * reference's class tables (fpa_threaded.h:1-93): NaN, SNaN and the tables' error
+ * entries (equal infinities, equal-sign TINY pairs) give 0 -- fpa_dispatch.h:109-153 */
 fpa_result fpa_sgn(int32_t *r, fpa_t x);
 fpa_result fpa_cmp(int32_t *r, fpa_t a, fpa_t b);
 

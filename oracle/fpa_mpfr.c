@@ -229,3 +229,43 @@ void mpfr_dot_array(uint32_t count, uint32_t precision, int mode, const uint32_t
     mpfr_clears(acc, p, x, y, (mpfr_ptr)0);
   }
 }
+
+/* ------------------------ set / sgn / cmp gold ------------------------ */
+/* r = mpfr_set(x) at r_precision (tests.c:580-605 draws its gold this way); ok[i] = 0
+ * for inputs MPFR cannot hold (TINY, SNAN). */
+void mpfr_set_array(uint32_t count, uint32_t r_precision, uint32_t x_precision, int mode, uint32_t *r_exps,
+                    uint32_t *r_limbs, uint8_t *ok, const uint32_t *x_exps, const uint32_t *x_limbs) {
+  int nr = (int)(r_precision / 32), nx = (int)(x_precision / 32);
+  wide_range();
+  mpfr_t r, x;
+  mpfr_init2(r, (mpfr_prec_t)r_precision);
+  mpfr_init2(x, (mpfr_prec_t)x_precision);
+  for (uint32_t i = 0; i < count; i++) {
+    ok[i] = mpf_from_fpa(x, x_exps[i], x_limbs + (size_t)i * nx, nx) == 0;
+    if (!ok[i]) continue;
+    mpfr_set(r, x, rnd_of(mode));
+    mpf_to_fpa(&r_exps[i], r_limbs + (size_t)i * nr, nr, r, 5, 0x7FFFFFFFll);
+  }
+  mpfr_clear(r);
+  mpfr_clear(x);
+}
+
+/* r[i] = mpfr_sgn(a) / mpfr_cmp(a, b) clamped to -1/0/+1; NaN operands give 0 (MPFR sets
+ * its erange flag and returns 0); ok[i] = 0 where an input is TINY or SNAN. */
+void mpfr_cmp_array(uint32_t count, uint32_t precision, const uint32_t *a_exps, const uint32_t *a_limbs,
+                    const uint32_t *b_exps, const uint32_t *b_limbs, int32_t *r, uint8_t *ok) {
+  int n = (int)(precision / 32);
+  wide_range();
+  mpfr_t a, b;
+  mpfr_init2(a, (mpfr_prec_t)precision);
+  mpfr_init2(b, (mpfr_prec_t)precision);
+  for (uint32_t i = 0; i < count; i++) {
+    ok[i] = mpf_from_fpa(a, a_exps[i], a_limbs + (size_t)i * n, n) == 0;
+    if (b_exps) ok[i] &= mpf_from_fpa(b, b_exps[i], b_limbs + (size_t)i * n, n) == 0;
+    if (!ok[i]) continue;
+    int c = b_exps ? mpfr_cmp(a, b) : mpfr_sgn(a);
+    r[i] = c > 0 ? 1 : (c < 0 ? -1 : 0);
+  }
+  mpfr_clear(a);
+  mpfr_clear(b);
+}

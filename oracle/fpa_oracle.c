@@ -37,6 +37,13 @@
  * Special-value action tables are restated as data from fpa_warp_add_sub.h:3-39,
  * fpa_warp_mul.h:1-12, fpa_warp_div.h:1-12, fpa_warp_sqrt.h:1-11.
  * Rounding decision restated from fpa_round (fpa_internal.h:62-88).
+ *
+ * set/sgn/cmp (fpa.h:57-61): sgn and cmp follow the reference's class tables
+ * (fpa_threaded.h:1-24; a table "error" entry yields 0, fpa_threaded.h:40-43, 69-70) and
+ * compare two same-sign values by exponent word, then mantissa (fpa_threaded.h:60-88).
+ * set (precision conversion) is unfinished in the reference (fpa_warp_set.h:134-199);
+ * its gold is mpfr_set (tests.c:580-605): the source rounded once to the target
+ * precision, special classes copied, and the add/sub/mul range rule on overflow.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -496,6 +503,64 @@ void orc_bn_divmod(uint32_t *q, uint32_t *rem, const uint32_t *u, int nu, const 
 }
 int orc_bn_isqrt(uint32_t *r, const uint32_t *x, int nx) { return bn_isqrt(r, x, nx); }
 void orc_bn_mul(uint32_t *r, const uint32_t *a, int na, const uint32_t *b, int nb) { bn_mul(r, a, na, b, nb); }
+
+/* ------------------------- set / sgn / cmp ------------------------- */
+#define T_CMP 2
+#define T_ERR 3
+static const int8_t sgn_tab[9] = { +1, -1, 0, -1, +1, -1, +1, T_ERR, T_ERR };
+static const int8_t cmp_tab[81] = {
+  /* a=+value */ T_CMP, +1, +1, +1, +1, +1, -1, T_ERR, T_ERR,
+  /*   -value */ -1, T_CMP, -1, -1, -1, +1, -1, T_ERR, T_ERR,
+  /*     zero */ -1, +1, 0, +1, -1, +1, -1, T_ERR, T_ERR,
+  /*   -1/inf */ -1, +1, -1, T_ERR, -1, +1, -1, T_ERR, T_ERR,
+  /*   +1/inf */ -1, +1, +1, +1, T_ERR, +1, -1, T_ERR, T_ERR,
+  /*     -inf */ -1, -1, -1, -1, -1, T_ERR, -1, T_ERR, T_ERR,
+  /*     +inf */ +1, +1, +1, +1, +1, +1, T_ERR, T_ERR, T_ERR,
+  /*      nan */ T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR,
+  /*     snan */ T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR, T_ERR,
+};
+
+/* the tables, for the tests (entry codes: -1, 0, +1, 2 = compare the values, 3 = error) */
+int orc_sgn_table(uint32_t cls) { return cls < 9 ? sgn_tab[cls] : -9; }
+int orc_cmp_table(uint32_t ca, uint32_t cb) { return ca < 9 && cb < 9 ? cmp_tab[ca * 9 + cb] : -9; }
+
+void orc_sgn_array(uint32_t count, const uint32_t *x_exps, int32_t *r) {
+  for (uint32_t i = 0; i < count; i++) {
+    int t = sgn_tab[classify(x_exps[i])];
+    r[i] = t == T_ERR ? 0 : t;
+  }
+}
+
+void orc_cmp_array(uint32_t count, uint32_t precision, const uint32_t *a_exps, const uint32_t *a_limbs,
+                   const uint32_t *b_exps, const uint32_t *b_limbs, int32_t *r) {
+  int n = (int)(precision / 32);
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t ea = a_exps[i], eb = b_exps[i];
+    int t = cmp_tab[classify(ea) * 9 + classify(eb)];
+    if (t == T_CMP) {
+      int sgn = (ea & 1u) ? -1 : 1;
+      if (ea != eb) t = ea > eb ? sgn : -sgn;
+      else t = sgn * bn_cmp(a_limbs + (size_t)i * n, b_limbs + (size_t)i * n, n);
+    }
+    r[i] = t == T_ERR ? 0 : t;
+  }
+}
+
+/* r (precision pr) = x (precision px) rounded in `mode` */
+void orc_set_array(uint32_t count, uint32_t r_precision, uint32_t x_precision, int mode, uint32_t *r_exps,
+                   uint32_t *r_limbs, const uint32_t *x_exps, const uint32_t *x_limbs) {
+  int nr = (int)(r_precision / 32), nx = (int)(x_precision / 32);
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t ex = x_exps[i];
+    uint32_t *rl = r_limbs + (size_t)i * nr;
+    if (ex <= C_SNAN) { set_special(&r_exps[i], rl, nr, ex); continue; }
+    int sign = (int)(ex & 1u);
+    int64_t scale = (int64_t)(ex >> 1) - ORC_BIAS - 32ll * nx;
+    int64_t e = round_to(rl, nr, x_limbs + (size_t)i * nx, nx, scale, 0, sign, mode);
+    r_exps[i] = pack_exp(e, sign, 5, 0x7FFFFFFFll);
+    if (r_exps[i] <= C_SNAN) memset(rl, 0, 4 * (size_t)nr);
+  }
+}
 
 /* Dot chains: r_j = fold over k of acc <- RN(acc + RN(x_k * y_k)) from an exact zero,
  * the same per-element op sequence MPFR gold uses (mpfr_mul then mpfr_add). */

@@ -166,41 +166,53 @@ def ref_lib():
     global _ref
     if _ref is None:
         _ref = ctypes.CDLL(REF_SO)
-        for name in ("add", "sub", "mul", "div", "sqrt", "neg"):
+        for name in ("add", "sub", "mul", "div", "sqrt", "neg", "sgn", "cmp"):
             f = getattr(_ref, "tests_generate_" + name)
             f.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
             f.restype = ctypes.POINTER(_Tests)
+        _ref.tests_generate_set.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        _ref.tests_generate_set.restype = ctypes.POINTER(_Tests)
         _ref.tests_free.argtypes = [ctypes.POINTER(_Tests)]
     return _ref
 
 
-def ref_generate(name, count, precision):
-    """Draw `count` cases from the reference's own generator (tests.c:727-925).
+def ref_generate(name, count, precision, source_precision=None):
+    """Draw `count` cases from the reference's own generator (tests.c:580-925).
 
-    Returns dict with a_exps, a_limbs, (b_exps, b_limbs,) gold_exps, gold_limbs.
+    Returns dict with a_exps, a_limbs, (b_exps, b_limbs,) gold_exps, (gold_limbs).  For
+    "set" the a operands have source_precision; for "sgn"/"cmp" gold_exps holds the
+    int32 results (mpfr_sgn / mpfr_cmp) and there are no gold limbs.
     The generator's GMP random state persists inside the loaded library, so successive
     calls continue the reference's deterministic sequence.
     """
-    t = getattr(ref_lib(), "tests_generate_" + name)(count, precision)
+    if name == "set":
+        t = ref_lib().tests_generate_set(count, precision, source_precision)
+    else:
+        t = getattr(ref_lib(), "tests_generate_" + name)(count, precision)
     if not t or t.contents.count != count or t.contents.precision != precision or t.contents.current != count:
         raise RuntimeError("reference harness returned an unusable tests_t")
     n = precision // 32
+    na = (source_precision if name == "set" else precision) // 32
     s = t.contents
 
     def arr(ptr, m):
         return np.ctypeslib.as_array(ptr, shape=(m,)).copy()
 
-    out = {"a_exps": arr(s.a_exps, count), "a_limbs": arr(s.a_limbs, count * n),
-           "gold_exps": arr(s.gold_exps, count), "gold_limbs": arr(s.gold_limbs, count * n)}
+    out = {"a_exps": arr(s.a_exps, count), "a_limbs": arr(s.a_limbs, count * na),
+           "gold_exps": arr(s.gold_exps, count)}
+    if name in ("sgn", "cmp"):
+        out["gold_exps"] = out["gold_exps"].view(np.int32)
+    else:
+        out["gold_limbs"] = arr(s.gold_limbs, count * n)
     if bool(s.b_exps):
         out["b_exps"] = arr(s.b_exps, count)
         out["b_limbs"] = arr(s.b_limbs, count * n)
     ref_lib().tests_free(t)
     # tests_set_mpfr leaves the limbs of zeros unset (tests.c:142-145): clear them
     for k in ("a", "b", "gold"):
-        if k + "_exps" in out:
+        if k + "_limbs" in out:
             z = out[k + "_exps"] == 2
-            out[k + "_limbs"].reshape(-1, n)[z] = 0
+            out[k + "_limbs"].reshape(len(z), -1)[z] = 0
     return out
 
 
@@ -233,3 +245,76 @@ def mpfr_dot(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode="rne", t
     f(count, precision, MODES[mode], _p(lengths), offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
       _p(_c(x_exps)), _p(_c(x_limbs)), _p(_c(y_exps)), _p(_c(y_limbs)), _p(r_exps), _p(r_limbs), threads)
     return r_exps, r_limbs
+
+
+# ----------------------------- set / sgn / cmp -----------------------------
+def sgn(x_exps):
+    x_exps = _c(x_exps)
+    r = np.zeros(len(x_exps), np.int32)
+    f = lib().orc_sgn_array
+    f.argtypes = [ctypes.c_uint32, _u32p, ctypes.POINTER(ctypes.c_int32)]
+    f(len(x_exps), _p(x_exps), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return r
+
+
+def cmp(precision, a_exps, a_limbs, b_exps, b_limbs):
+    a_exps, a_limbs, b_exps, b_limbs = _c(a_exps), _c(a_limbs), _c(b_exps), _c(b_limbs)
+    r = np.zeros(len(a_exps), np.int32)
+    f = lib().orc_cmp_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, ctypes.POINTER(ctypes.c_int32)]
+    f(len(a_exps), precision, _p(a_exps), _p(a_limbs), _p(b_exps), _p(b_limbs),
+      r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return r
+
+
+def set_precision(r_precision, x_precision, x_exps, x_limbs, mode="rne"):
+    x_exps, x_limbs = _c(x_exps), _c(x_limbs)
+    count = len(x_exps)
+    re = np.zeros(count, np.uint32)
+    rl = np.zeros(count * (r_precision // 32), np.uint32)
+    f = lib().orc_set_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, _u32p, _u32p, _u32p]
+    f(count, r_precision, x_precision, MODES[mode], _p(re), _p(rl), _p(x_exps), _p(x_limbs))
+    return re, rl
+
+
+def mpfr_set(r_precision, x_precision, x_exps, x_limbs, mode="rne"):
+    """mpfr_set gold; returns (r_exps, r_limbs, ok)."""
+    x_exps, x_limbs = _c(x_exps), _c(x_limbs)
+    count = len(x_exps)
+    re = np.zeros(count, np.uint32)
+    rl = np.zeros(count * (r_precision // 32), np.uint32)
+    ok = np.zeros(count, np.uint8)
+    f = lib().mpfr_set_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, _u32p,
+                  ctypes.POINTER(ctypes.c_uint8), _u32p, _u32p]
+    f(count, r_precision, x_precision, MODES[mode], _p(re), _p(rl), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+      _p(x_exps), _p(x_limbs))
+    return re, rl, ok.astype(bool)
+
+
+def mpfr_cmp(precision, a_exps, a_limbs, b_exps=None, b_limbs=None):
+    """mpfr_cmp (or mpfr_sgn when b is None) clamped to -1/0/1; returns (r, ok)."""
+    a_exps, a_limbs = _c(a_exps), _c(a_limbs)
+    count = len(a_exps)
+    r = np.zeros(count, np.int32)
+    ok = np.zeros(count, np.uint8)
+    f = lib().mpfr_cmp_array
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, ctypes.POINTER(ctypes.c_int32),
+                  ctypes.POINTER(ctypes.c_uint8)]
+    f(count, precision, _p(a_exps), _p(a_limbs), _p(_c(b_exps)) if b_exps is not None else None,
+      _p(_c(b_limbs)) if b_limbs is not None else None, r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+      ok.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return r, ok.astype(bool)
+
+
+def sgn_table(cls):
+    f = lib().orc_sgn_table
+    f.argtypes = [ctypes.c_uint32]
+    return f(cls)
+
+
+def cmp_table(ca, cb):
+    f = lib().orc_cmp_table
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    return f(ca, cb)

@@ -5,7 +5,11 @@
   harness's MPFR gold (round to nearest even).
 * mpfr_<op>_<p>.npz -- config-2-style operands (zeros, near-cancellation pairs) from the
   package generator at precisions outside the reference's range, with MPFR gold.
-Usage: python tests/golden/make_golden.py
+* setcmp/ref_set_<p>_<src>.npz, setcmp/ref_sgn_<p>.npz, setcmp/ref_cmp_<p>.npz -- the
+  reference generator's set / sgn / cmp cases (tests.c:580-699) with its MPFR gold
+  (mpfr_set in round to nearest even, mpfr_sgn, mpfr_cmp).
+Usage: python tests/golden/make_golden.py            (element-wise ops)
+       python tests/golden/make_golden.py --setcmp   (set / sgn / cmp only)
 """
 import os
 import sys
@@ -40,5 +44,23 @@ def main():
     print("fixtures written to", HERE)
 
 
+def main_setcmp():
+    O.build()
+    out = os.path.join(HERE, "setcmp")
+    os.makedirs(out, exist_ok=True)
+    for p, src in ((1024, 3072), (3072, 1024), (2048, 2048), (1024, 8192), (8192, 1024)):
+        g = O.ref_generate("set", 48, p, src)
+        np.savez_compressed(os.path.join(out, f"ref_set_{p}_{src}.npz"), precision=np.uint32(p),
+                            source_precision=np.uint32(src), **g)
+    for name in ("sgn", "cmp"):
+        for p in (1024, 4096):
+            g = O.ref_generate(name, 64, p)
+            np.savez_compressed(os.path.join(out, f"ref_{name}_{p}.npz"), precision=np.uint32(p), **g)
+    print("fixtures written to", out)
+
+
 if __name__ == "__main__":
-    main()
+    if "--setcmp" in sys.argv:
+        main_setcmp()
+    else:
+        main()
