@@ -4,19 +4,70 @@
  * the reference's fpa_test.c test_<op> functions do (fpa_test.c:207-327).
  * tests_compare_results exits(1) on the first mismatch, so the exit status is the verdict.
  *
- * usage: drop_in <op> <count> <precision>     op = add|sub|mul|div|sqrt
+ * usage: drop_in <op> <count> <precision> [source_precision]
+ *        op = add|sub|mul|div|sqrt|sgn|cmp, or set (needs source_precision)
+ * sgn/cmp write into a device int32 array, as fpa_test.c does with cudaMalloc
+ * (fpa_test.c:144-155); here hipMalloc.
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
+#include <hip/hip_runtime_api.h>
 #include "fpa.h"
 #include "tests.h"
 
+static int sgn_cmp(const char *op, uint32_t count, uint32_t precision) {
+  tests_t *t = !strcmp(op, "sgn") ? tests_generate_sgn(count, precision) : tests_generate_cmp(count, precision);
+  if (!t) return 3;
+  fpa_t a, b;
+  int32_t *dr;
+  if (fpa_init(a, count, precision) != fpa_success || hipMalloc((void **)&dr, 4 * (size_t)count) != hipSuccess) return 4;
+  fpa_copy_array_to_device(a, t->a_exps, t->a_limbs);
+  fpa_result rc;
+  if (!strcmp(op, "sgn")) rc = fpa_sgn(dr, a);
+  else {
+    fpa_init(b, count, precision);
+    fpa_copy_array_to_device(b, t->b_exps, t->b_limbs);
+    rc = fpa_cmp(dr, a, b);
+  }
+  if (rc != fpa_success) { fprintf(stderr, "fpa_%s returned %d\n", op, rc); return 5; }
+  if (hipMemcpy(t->r_exps, dr, 4 * (size_t)count, hipMemcpyDeviceToHost) != hipSuccess) return 6;
+  tests_compare_results(t);
+  printf("%s tests passed: %u values at %u bits\n", op, count, precision);
+  tests_free(t);
+  fpa_clear(a);
+  if (strcmp(op, "sgn")) fpa_clear(b);
+  hipFree(dr);
+  return 0;
+}
+
+static int set(uint32_t count, uint32_t precision, uint32_t source) {
+  tests_t *t = tests_generate_set(count, precision, source);
+  if (!t) return 3;
+  fpa_t r, x;
+  if (fpa_init(r, count, precision) != fpa_success || fpa_init(x, count, source) != fpa_success) return 4;
+  fpa_copy_array_to_device(x, t->a_exps, t->a_limbs);
+  fpa_result rc = fpa_set(r, x, fpa_round_ties_to_even);
+  if (rc != fpa_success) { fprintf(stderr, "fpa_set returned %d\n", rc); return 5; }
+  fpa_copy_array_to_host(r, t->r_exps, t->r_limbs);
+  tests_compare_results(t);
+  printf("set tests passed: %u values, %u -> %u bits\n", count, source, precision);
+  tests_free(t);
+  fpa_clear(r);
+  fpa_clear(x);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s op count precision\n", argv[0]); return 2; }
+  if (argc != 4 && argc != 5) { fprintf(stderr, "usage: %s op count precision [source]\n", argv[0]); return 2; }
   const char *op = argv[1];
   uint32_t count = (uint32_t)atoi(argv[2]), precision = (uint32_t)atoi(argv[3]);
+  if (!strcmp(op, "sgn") || !strcmp(op, "cmp")) return sgn_cmp(op, count, precision);
+  if (!strcmp(op, "set")) {
+    if (argc != 5) { fprintf(stderr, "set needs a source precision\n"); return 2; }
+    return set(count, precision, (uint32_t)atoi(argv[4]));
+  }
   tests_t *t;
   if (!strcmp(op, "add")) t = tests_generate_add(count, precision);
   else if (!strcmp(op, "sub")) t = tests_generate_sub(count, precision);
