@@ -26,7 +26,7 @@ struct ScratchWords {
   static constexpr int n = K * L;
   static constexpr int raw = OP == 0 ? 0
                            : OP <= 2 ? n + 1
-                           : OP == 3 ? 2 * L * mul_digits_per_lane(K)
+                           : OP == 3 ? mul_lds_words(K, L)
                            : OP == 4 ? n + 1
                            : 2 * n + 1;
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
@@ -49,7 +49,7 @@ __global__ FPA_KERNEL_ATTRS void fpa_kernel(fpa_arguments a) {
   if constexpr (OP == 0) op_neg<K, L>(a, s, g);
   else if constexpr (OP == 1) op_addsub<K, L>(a, s, g, lds, false);
   else if constexpr (OP == 2) op_addsub<K, L>(a, s, g, lds, true);
-  else if constexpr (OP == 3) op_mul<K, L>(a, s, g, lds);
+  else if constexpr (OP == 3) op_mul<K, L>(a, s, g, lds, SW);
   else if constexpr (OP == 4) op_div<K, L>(a, s, g, lds);
   else op_sqrt<K, L>(a, s, g, lds);
 }

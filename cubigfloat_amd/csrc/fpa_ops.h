@@ -305,6 +305,21 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
 constexpr int MUL_RADIX = 27;
 
 __host__ __device__ constexpr int mul_digits_per_lane(int K) { return (32 * K + MUL_RADIX - 1) / MUL_RADIX; }
+/* digit chunks in LDS are padded to whole 16-byte rows */
+__host__ __device__ constexpr int mul_digits_padded(int K) { return (mul_digits_per_lane(K) + 3) / 4 * 4; }
+/* groups of 2..16 lanes sit inside one 16-lane DPP row: their window shift is a DPP
+ * row shift and their broadcasts come from LDS */
+__host__ __device__ constexpr bool mul_row_path(int L) { return L >= 2 && L <= 16; }
+/* LDS words per value: 64-bit low columns (later the digits), plus a's digit chunks on
+ * the row path */
+__host__ __device__ constexpr int mul_lds_words(int K, int L) {
+  return 2 * L * mul_digits_per_lane(K) + (mul_row_path(L) ? L * mul_digits_padded(K) : 0);
+}
+
+/* x of lane+1 within this lane's 16-lane row (DPP row_shl:1); the row's lane 15 gets 0 */
+__device__ __forceinline__ uint32_t dpp_row_next(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x101, 0xF, 0xF, true);
+}
 
 /* 32 bits at bit offset o of a digit string (radix 2^27) held in LDS, nd digits */
 __device__ __forceinline__ uint32_t digits_bits(const uint32_t *d, int nd, int o) {
@@ -342,18 +357,27 @@ __device__ __forceinline__ uint32_t add_carry_digits(uint32_t (&dg)[KD], uint64_
   return (uint32_t)c;
 }
 
-/* (re, R) = a * b; lds: 2 * L * KD words (64-bit low columns, then digits) */
-template <int K, int L>
-__device__ void core_mul(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&Aw)[K],
-                         uint32_t eb, const uint32_t (&Bw)[K], uint32_t &re, uint32_t (&R)[K]) {
+/* (re, R) = a * b; lds: mul_lds_words(K, L) words (64-bit low columns, then digits; a's
+ * digit chunks after them on the row path).  group_stride = LDS words from this group's
+ * base to the next group's in the same wave.
+ * kLockstep: every group of the wave calls this together (the element-wise kernel);
+ * special-class operands then run the digit rounds too and take their class result at
+ * the end, so the row path can let a group's top lane read its neighbour group and
+ * correct for it once.  Without it (dot chains: groups run chains of different lengths)
+ * the top lane's incoming value is masked each round instead. */
+template <int K, int L, bool kLockstep>
+__device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int mode, uint32_t ea,
+                         const uint32_t (&Aw)[K], uint32_t eb, const uint32_t (&Bw)[K], uint32_t &re,
+                         uint32_t (&R)[K]) {
   constexpr int n = K * L;
   constexpr int KD = mul_digits_per_lane(K);
+  constexpr int KDP = mul_digits_padded(K);
   constexpr int ND = KD * L;               /* digits per operand */
   constexpr uint32_t M = (1u << MUL_RADIX) - 1;
   static_assert(2 * L <= 64, "low and high chunk carries share one 64-bit mask");
   static_assert(ND < 4096, "column sums of 27-bit digits must stay below 2^64");
-  uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
-  if (act != FPA_A_MUL) {
+  const uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  if (!kLockstep && act != FPA_A_MUL) {
     re = act;
 #pragma unroll
     for (int i = 0; i < K; i++) R[i] = 0;
@@ -375,32 +399,84 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
     wsync();
   }
 
+  /* Product scanning: round r multiplies a's chunk r (broadcast to the group) by this
+   * lane's chunk of b into a window of 2*KD 64-bit column sums (column chunks r+gl and
+   * r+gl+1).  Lane 0's low half is then complete and goes to LDS; every lane's new low
+   * half is its high half plus the next lane's low half. */
   uint64_t W[2 * KD];
 #pragma unroll
   for (int i = 0; i < 2 * KD; i++) W[i] = 0;
   uint64_t *low = reinterpret_cast<uint64_t *>(lds);   /* ND 64-bit low column sums */
-  for (int r = 0; r < L; r++) {
+  if constexpr (mul_row_path(L)) {
+    uint32_t *adig = lds + 2 * ND;                      /* a's digit chunks, KDP apart */
 #pragma unroll
-    for (int i = 0; i < KD; i++) {
-      const uint32_t ai = g.bcast(A[i], r);
+    for (int k = 0; k < KD; k++) adig[g.gl * KDP + k] = A[k];
+    wsync();
+    for (int r = 0; r < L; r++) {
+      uint32_t ar[KDP];
 #pragma unroll
-      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ai * B[j];
+      for (int q = 0; q < KDP / 4; q++) {
+        const uint4 t = reinterpret_cast<const uint4 *>(adig + r * KDP)[q];
+        ar[4 * q] = t.x; ar[4 * q + 1] = t.y; ar[4 * q + 2] = t.z; ar[4 * q + 3] = t.w;
+      }
+#pragma unroll
+      for (int i = 0; i < KD; i++) {
+#pragma unroll
+        for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ar[i] * B[j];
+      }
+      if (g.gl == 0) {
+#pragma unroll
+        for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
+      }
+      /* Lockstep: a group's top lane reads lane 0 of the next group in its row (not
+       * zero); that value is the next group's complete chunk r, and it ends up, unchanged,
+       * in lane r's final window, where it is subtracted below. */
+      const bool top = !kLockstep && g.gl == L - 1;
+#pragma unroll
+      for (int i = 0; i < KD; i++) {
+        uint32_t lo = dpp_row_next((uint32_t)W[i]);
+        uint32_t hi = dpp_row_next((uint32_t)(W[i] >> 32));
+        if (!kLockstep) {
+          lo = top ? 0u : lo;
+          hi = top ? 0u : hi;
+        }
+        W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
+        W[KD + i] = 0;
+      }
     }
-    /* group lane 0's low window: columns r*KD .. r*KD+KD-1 are complete */
-    if (g.gl == 0) {
+  } else {
+    for (int r = 0; r < L; r++) {
 #pragma unroll
-      for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
-    }
-    /* next window = own high half + next lane's low half */
+      for (int i = 0; i < KD; i++) {
+        const uint32_t ai = g.bcast(A[i], r);
 #pragma unroll
-    for (int i = 0; i < KD; i++) {
-      uint32_t lo = g.from_next((uint32_t)W[i]);
-      uint32_t hi = g.from_next((uint32_t)(W[i] >> 32));
-      W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
-      W[KD + i] = 0;
+        for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ai * B[j];
+      }
+      /* group lane 0's low window: columns r*KD .. r*KD+KD-1 are complete */
+      if (g.gl == 0) {
+#pragma unroll
+        for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
+      }
+      /* next window = own high half + next lane's low half */
+#pragma unroll
+      for (int i = 0; i < KD; i++) {
+        uint32_t lo = g.from_next((uint32_t)W[i]);
+        uint32_t hi = g.from_next((uint32_t)(W[i] >> 32));
+        W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
+        W[KD + i] = 0;
+      }
     }
   }
   wsync();
+  if constexpr (kLockstep && mul_row_path(L) && L < 16) {
+    /* groups whose top lane is not the last lane of its DPP row took in the next group's
+     * complete chunks: lane j's final window holds that group's chunk j on top */
+    if (((g.gbase + L) & 15) != 0) {
+      const uint64_t *nlow = reinterpret_cast<const uint64_t *>(lds + group_stride);
+#pragma unroll
+      for (int i = 0; i < KD; i++) W[i] -= nlow[g.gl * KD + i];
+    }
+  }
 
   /* normalise: chain of 2L chunks, low chunks 0..L-1 then high chunks 0..L-1 */
   uint32_t dl[KD], dh[KD];
@@ -463,14 +539,19 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
   if (round_mantissa<K, L>(g, H, mode, sign, rw, sticky)) E += 1;
   re = pack_exp(E, sign, 5, 0x7FFFFFFFll);
   chunk_select(R, true, H, H);
+  if (kLockstep && act != FPA_A_MUL) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = 0;
+  }
 }
 
 template <int K, int L>
-__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, int group_stride) {
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
-  core_mul<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  core_mul<K, L, true>(g, lds, group_stride, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
   store_result(a, s, g.gl, re, R);
 }
 
