@@ -321,14 +321,40 @@ __device__ __forceinline__ uint32_t dpp_row_next(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x101, 0xF, 0xF, true);
 }
 
-/* 32 bits at bit offset o of a digit string (radix 2^27) held in LDS, nd digits */
-__device__ __forceinline__ uint32_t digits_bits(const uint32_t *d, int nd, int o) {
-  int k = o / MUL_RADIX, off = o - k * MUL_RADIX;
-  uint64_t t = 0;
-  if (k < nd) t = d[k] >> off;
-  if (k + 1 < nd) t |= (uint64_t)d[k + 1] << (MUL_RADIX - off);
-  if (k + 2 < nd) t |= (uint64_t)d[k + 2] << (2 * MUL_RADIX - off);
+/* 32 bits at bit offset o >= 0 of a digit string (radix 2^27) held in LDS; the two words
+ * past its top digit must read as zero */
+__device__ __forceinline__ uint32_t digits_bits(const uint32_t *d, int o) {
+  const int k = o / MUL_RADIX, off = o - k * MUL_RADIX;
+  uint64_t t = d[k] >> off;
+  t |= (uint64_t)d[k + 1] << (MUL_RADIX - off);
+  t |= (uint64_t)d[k + 2] << (2 * MUL_RADIX - off);
   return (uint32_t)t;
+}
+
+/* This lane's radix-2^27 digits of an n = K*L word value: digit k = bits
+ * [27*(gl*KD + k), +27).  The value is staged in LDS with zero words past its top, so
+ * each digit is one two-word read and a funnel shift. */
+template <int K, int L, int KD>
+__device__ __forceinline__ void recode_digits(const Group<L> &g, uint32_t *lds, const uint32_t (&X)[K],
+                                              uint32_t (&D)[KD]) {
+  constexpr int n = K * L;
+  constexpr int PADW = (MUL_RADIX * KD * L + 31) / 32 + 1 - n;   /* words [n, n+PADW) read as 0 */
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+  static_assert(PADW >= 1, "digits cover the value");
+  lds_store(lds + g.gl * K, X);
+  if (g.gl == L - 1) {
+#pragma unroll
+    for (int i = 0; i < PADW; i++) lds[n + i] = 0;
+  }
+  wsync();
+  const int b = MUL_RADIX * KD * g.gl;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    const int pos = b + MUL_RADIX * k;
+    const int w = pos >> 5;
+    D[k] = __builtin_amdgcn_alignbit(lds[w + 1], lds[w], (uint32_t)pos & 31u) & M;
+  }
+  wsync();
 }
 
 /* local radix-2^27 normalisation of KD column sums with carry-in c; returns the carry out */
@@ -375,7 +401,11 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
   constexpr int ND = KD * L;               /* digits per operand */
   constexpr uint32_t M = (1u << MUL_RADIX) - 1;
   static_assert(2 * L <= 64, "low and high chunk carries share one 64-bit mask");
-  static_assert(ND < 4096, "column sums of 27-bit digits must stay below 2^64");
+  /* a column sums at most ND products below 2^54; on the lockstep row path a window can
+   * also hold one chunk of the neighbour group on top of its own */
+  static_assert(ND <= 1024, "column sums of 27-bit digits must stay below 2^64");
+  static_assert(!(kLockstep && mul_row_path(L) && L < 16) || ND <= 512,
+                "own column sum plus one neighbour chunk must stay below 2^64");
   const uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
   if (!kLockstep && act != FPA_A_MUL) {
     re = act;
@@ -386,18 +416,8 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
 
   /* recode both operands into radix-2^27 digit chunks via LDS */
   uint32_t A[KD], B[KD];
-  {
-    lds_store(lds + g.gl * K, Aw);
-    wsync();
-#pragma unroll
-    for (int k = 0; k < KD; k++) A[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
-    wsync();
-    lds_store(lds + g.gl * K, Bw);
-    wsync();
-#pragma unroll
-    for (int k = 0; k < KD; k++) B[k] = lds_bits(lds, n, MUL_RADIX * (g.gl * KD + k)) & M;
-    wsync();
-  }
+  recode_digits<K, L, KD>(g, lds, Aw, A);
+  recode_digits<K, L, KD>(g, lds, Bw, B);
 
   /* Product scanning: round r multiplies a's chunk r (broadcast to the group) by this
    * lane's chunk of b into a window of 2*KD 64-bit column sums (column chunks r+gl and
@@ -440,7 +460,11 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
           lo = top ? 0u : lo;
           hi = top ? 0u : hi;
         }
-        W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
+        /* one add-with-carry per half, so each row shift can fold into its add */
+        unsigned int c, c2;
+        const uint32_t nlo = __builtin_addc((uint32_t)W[KD + i], lo, 0u, &c);
+        const uint32_t nhi = __builtin_addc((uint32_t)(W[KD + i] >> 32), hi, c, &c2);
+        W[i] = ((uint64_t)nhi << 32) | nlo;
         W[KD + i] = 0;
       }
     }
@@ -504,12 +528,17 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
     uint64_t X = G | P;
     uint64_t S = X + G;                      /* 2L <= 64 bits; the value's top carry is zero */
     uint64_t C = S ^ X ^ G;
-    add_carry_digits<KD>(dl, (C >> g.gl) & 1u);
-    add_carry_digits<KD>(dh, (C >> (L + g.gl)) & 1u);
+    const uint32_t cl1 = (uint32_t)(C >> g.gl) & 1u, ch1 = (uint32_t)(C >> (L + g.gl)) & 1u;
+    /* a 1-bit carry needs a chunk of all-ones digits: rare, so test the whole wave */
+    if (__builtin_expect(__ballot((cl1 | ch1) != 0) != 0, 0)) {
+      add_carry_digits<KD>(dl, cl1);
+      add_carry_digits<KD>(dh, ch1);
+    }
   }
-  /* digits to LDS: [0, ND) low, [ND, 2ND) high */
+  /* digits to LDS: [0, ND) low, [ND, 2ND) high, then two zero words */
 #pragma unroll
   for (int i = 0; i < KD; i++) { lds[g.gl * KD + i] = dl[i]; lds[ND + g.gl * KD + i] = dh[i]; }
+  if (g.gl == L - 1) { lds[2 * ND] = 0; lds[2 * ND + 1] = 0; }
   wsync();
 
   /* P = A*B < 2^(2p); mantissa = top p bits from bit `base`, round word below, sticky */
@@ -519,8 +548,8 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
   const int base = full ? p : p - 1;
   uint32_t H[K];
 #pragma unroll
-  for (int i = 0; i < K; i++) H[i] = digits_bits(lds, 2 * ND, base + 32 * (g.gl * K + i));
-  uint32_t rw = digits_bits(lds, 2 * ND, base - 32);
+  for (int i = 0; i < K; i++) H[i] = digits_bits(lds, base + 32 * (g.gl * K + i));
+  uint32_t rw = digits_bits(lds, base - 32);
   uint32_t loc = 0;
   {
     const int lim = base - 32;               /* sticky covers bits [0, lim) */
