@@ -154,9 +154,11 @@ def measured_traffic():
 
 def kernels_only(F, hip, args):
     """Short run for hardware-counter passes: neg (a pure copy of known bytes), mul and add
-    at 4096 bits over the same 2^20-value workload, one warmup and `steps` launches each."""
+    (or the ops named by --only) at 4096 bits over the same 2^20-value workload, one
+    warmup and `steps` launches each."""
     res = []
-    for op in ("neg", "mul", "add"):
+    ops = [o for o in args.only.split(",") if o] or ["neg", "mul", "add"]
+    for op in ops:
         print(f"[bench] kernels-only {op} 4096 x{args.count}", file=sys.stderr, flush=True)
         r, _, _ = time_op(F, hip, op, 4096, args.count, args.steps, 1, 2, 2, 16)
         r["algorithmic_bytes_per_launch"] = algorithmic(op, 4096)[1] * args.count

@@ -282,10 +282,13 @@ struct Group {
   template <int K>
   __device__ __forceinline__ uint32_t add_lane_words(uint32_t (&x)[K], uint32_t w) const {
     uint32_t c = chunk_add_word(x, w);
+    /* A chunk passes a carry on only if adding w overflowed all its limbs: rare.  When no
+     * lane of the wave did, no carry enters any lane and none leaves any value (the
+     * lookahead below would give all zeros), so the resolution is skipped. */
+    if (__builtin_expect(__ballot(c != 0) == 0, 1)) return 0u;
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(x) == 0xFFFFFFFFu, 0u, cout);
-    /* a chunk passes a carry on only if it overflowed: rare, so test the whole wave */
-    if (__builtin_expect(__ballot(ci != 0) != 0, 0)) chunk_add_word(x, ci);
+    if (__ballot(ci != 0) != 0) chunk_add_word(x, ci);
     return cout;
   }
 
@@ -293,9 +296,12 @@ struct Group {
   template <int K>
   __device__ __forceinline__ uint32_t sub_lane_words(uint32_t (&x)[K], uint32_t w) const {
     uint32_t c = chunk_sub_word(x, w);
+    /* a chunk passes a borrow on only if its limbs above word 0 were all zero: rare; with
+     * no such lane in the wave nothing propagates (see add_lane_words) */
+    if (__builtin_expect(__ballot(c != 0) == 0, 1)) return 0u;
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(x) == 0u, 0u, bout);
-    if (__builtin_expect(__ballot(bi != 0) != 0, 0)) chunk_sub_word(x, bi);
+    if (__ballot(bi != 0) != 0) chunk_sub_word(x, bi);
     return bout;
   }
 
