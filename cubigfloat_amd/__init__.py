@@ -237,6 +237,19 @@ def set_variant(op, precision, variant):
     return lib.fpa_kernel_set_variant(KERNELS[op], precision, variant)
 
 
+lib.fpa_kernel_set_small_variant.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]
+lib.fpa_kernel_get_variant_for_count.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64]
+
+
+def set_small_variant(op, precision, variant, max_count):
+    """Variant for launches of at most max_count values (max_count 0 removes it)."""
+    return lib.fpa_kernel_set_small_variant(KERNELS[op], precision, variant, max_count)
+
+
+def get_variant_for_count(op, precision, count):
+    return lib.fpa_kernel_get_variant_for_count(KERNELS[op], precision, count)
+
+
 def get_variant(op, precision):
     return lib.fpa_kernel_get_variant(KERNELS[op], precision)
 

@@ -5,7 +5,11 @@ value, K*L = precision/32).  For every (op, precision) this times every shape on
 same synthetic operands (fpa_workload_generate kind 2: e ~ U[-1024, 1024], 1% zeros, 5%
 near-cancellation pairs), checks that every shape returns bit-identical results, and
 writes the fastest shape as one "op precision K L" line (op 6 = the dot-chain kernel,
-timed on chains of U[8, 512] elements).  libfpa.so applies the file
+timed on chains of U[8, 512] elements).  It then times every shape again on a small
+launch (16,384 values by default, about the size of one segment of a 2^18-op mixed batch):
+such a launch cannot fill the chip and the shape with the shortest per-value latency
+wins.  Where that shape differs, a fifth field marks an extra "op precision K L max_count"
+line, used for launches of at most max_count values.  libfpa.so applies the file
 when it is loaded (tuning.txt beside the library, or $FPA_TUNING_FILE), so the fpa.h
 calls, the batch graphs, the server and the dot chains all use the tuned shapes.
 
@@ -22,7 +26,7 @@ import time
 import numpy as np
 
 from . import (DotPlan, FpaArguments, FpaArray, KERNELS, OPS, PRECISIONS, generate, get_variant, lib,
-               set_variant, variants)
+               set_small_variant, set_variant, variants)
 
 lib.fpa_kernel_time.argtypes = [ctypes.c_int, ctypes.POINTER(FpaArguments), ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
@@ -144,24 +148,36 @@ def tune_one(op, precision, count=None, seed=1, log=None):
                 x.close()
 
 
-def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None):
-    """Returns {(op, precision): ((K, L), timings, count)}; also applies each choice."""
-    out = {}
+def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None, small_count=16384):
+    """Returns ({(op, precision): ((K, L), timings, count)}, {same for small launches where
+    the small-launch winner differs}); also applies each choice."""
+    out, small = {}, {}
     for op in ops:
         for p in precisions:
             best, timings, count = tune_one(op, p, log=log)
             out[(op, p)] = (best, timings, count)
-            set_variant(op, p, [s for s in variants(op, p)].index(best))
-    return out
+            set_variant(op, p, variants(op, p).index(best))
+            if small_count and op != "dot":
+                sbest, stimings, scount = tune_one(op, p, count=small_count, log=log)
+                if sbest != best:
+                    small[(op, p)] = (sbest, stimings, scount)
+                    set_small_variant(op, p, variants(op, p).index(sbest), small_count)
+    return out, small
 
 
-def write_table(path, table):
+def write_table(path, table, small=None):
+    small = small or {}
     lines = ["# kernel shapes chosen by cubigfloat_amd.autotune, " + time.strftime("%Y-%m-%d"),
-             "# op precision K L   (per-shape mean launch time; * = chosen)"]
+             "# op precision K L [max_count]   (per-shape mean launch time; * = chosen)"]
     for (op, p), (best, timings, count) in sorted(table.items(), key=lambda kv: (KERNELS[kv[0][0]], kv[0][1])):
         parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == best else "") for k, l, ms in timings)
         lines.append(f"# {op} {p}, {count} {'elements' if op == 'dot' else 'values'}: {parts}")
         lines.append(f"{KERNELS[op]} {p} {best[0]} {best[1]}")
+        if (op, p) in small:
+            sbest, stimings, scount = small[(op, p)]
+            parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == sbest else "") for k, l, ms in stimings)
+            lines.append(f"# {op} {p}, small launches of {scount} values: {parts}")
+            lines.append(f"{KERNELS[op]} {p} {sbest[0]} {sbest[1]} {scount}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -171,6 +187,8 @@ def main():
     ap.add_argument("--out", required=True, help="tuning table to write")
     ap.add_argument("--ops", default=",".join(TUNED_OPS))
     ap.add_argument("--precisions", default=",".join(str(p) for p in PRECISIONS))
+    ap.add_argument("--small-count", type=int, default=16384,
+                    help="values per small launch timed for the small-launch table (0: skip)")
     args = ap.parse_args()
     ops = [o for o in args.ops.split(",") if o]
     precs = [int(p) for p in args.precisions.split(",") if p]
@@ -178,9 +196,9 @@ def main():
         if op not in KERNELS:
             sys.exit(f"unknown op {op}")
     log = lambda s: print(s, flush=True)
-    table = tune(ops, precs, log=log)
-    write_table(args.out, table)
-    log(f"wrote {len(table)} entries to {args.out}")
+    table, small = tune(ops, precs, log=log, small_count=args.small_count)
+    write_table(args.out, table, small)
+    log(f"wrote {len(table)} entries and {len(small)} small-launch entries to {args.out}")
 
 
 if __name__ == "__main__":

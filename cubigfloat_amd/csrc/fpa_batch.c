@@ -110,8 +110,6 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
       b->sh[s].op = ops[i];
       b->sh[s].precision = precisions[i];
       b->sh[s].mode = modes[i];
-      b->sh[s].variant = fpa_kernel_get_variant(ops[i], precisions[i]);
-      fpa_kernel_variant_shape(ops[i], precisions[i], b->sh[s].variant, &b->sh[s].K, &b->sh[s].L);
       b->nseg++;
     }
     b->op_seg[i] = s;
@@ -135,6 +133,8 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
   FPA_HIP(hipHostMalloc((void **)&b->mirror, b->pool_bytes, hipHostMallocDefault));
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
+    g->variant = fpa_kernel_get_variant_for_count(g->op, g->precision, g->count);
+    fpa_kernel_variant_shape(g->op, g->precision, g->variant, &g->K, &g->L);
     fpa_arguments *a = &g->args;
     a->count = g->count;
     a->precision = g->precision;
@@ -332,7 +332,8 @@ fpa_result fpa_dot_plan_run(fpa_dot_plan_t p, fpa_t r, fpa_t x, fpa_t y, fpa_rou
   d.offsets = p->offsets_dev;
   d.order = p->order_dev;
   int K, L;
-  fpa_kernel_variant_shape(FPA_KERNEL_DOT, x->precision, fpa_kernel_get_variant(FPA_KERNEL_DOT, x->precision), &K, &L);
+  fpa_kernel_variant_shape(FPA_KERNEL_DOT, x->precision,
+                           fpa_kernel_get_variant_for_count(FPA_KERNEL_DOT, x->precision, p->count), &K, &L);
   int rc = fpa_impl_dot_launch(&d, K, L, stream);
   if (rc != 0) { fprintf(stderr, "dot kernel launch failed: %d\n", rc); exit(1); }
   return fpa_success;

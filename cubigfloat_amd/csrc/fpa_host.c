@@ -59,6 +59,10 @@ static const int prec_n[] = { 8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 512 };
 
 /* dispatch table: variant index per (op, precision); 0 = first compiled shape */
 static int variant_tab[FPA_OP_COUNT + 1][NPREC];   /* + FPA_KERNEL_DOT */
+/* optional shape for launches of at most small_max[op][p] values (0 = none): small
+ * launches cannot fill the chip, so the shape with the shortest per-value latency wins */
+static int variant_small[FPA_OP_COUNT + 1][NPREC];
+static uint32_t small_max[FPA_OP_COUNT + 1][NPREC];
 static void *cur_stream = NULL;
 
 static int prec_index(uint32_t precision) {
@@ -106,12 +110,35 @@ int fpa_kernel_get_variant(int op, uint32_t precision) {
   return variant_tab[op][p];
 }
 
-int fpa_kernel_set_variant(int op, uint32_t precision, int variant) {
+static int set_large_variant(int op, uint32_t precision, int variant) {
   int p = prec_index(precision);
   if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
   if (variant < 0 || variant >= fpa_kernel_variants(op, precision)) return -1;
   variant_tab[op][p] = variant;
   return 0;
+}
+
+/* an explicit choice applies to every launch size: it also drops a small-launch variant */
+int fpa_kernel_set_variant(int op, uint32_t precision, int variant) {
+  if (set_large_variant(op, precision, variant) != 0) return -1;
+  small_max[op][prec_index(precision)] = 0;
+  return 0;
+}
+
+int fpa_kernel_set_small_variant(int op, uint32_t precision, int variant, uint32_t max_count) {
+  int p = prec_index(precision);
+  if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
+  if (variant < 0 || variant >= fpa_kernel_variants(op, precision)) return -1;
+  variant_small[op][p] = variant;
+  small_max[op][p] = max_count;
+  return 0;
+}
+
+int fpa_kernel_get_variant_for_count(int op, uint32_t precision, uint64_t count) {
+  int p = prec_index(precision);
+  if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
+  if (small_max[op][p] && count <= small_max[op][p]) return variant_small[op][p];
+  return variant_tab[op][p];
 }
 
 int fpa_kernel_load_tuning(const char *path) {
@@ -121,14 +148,19 @@ int fpa_kernel_load_tuning(const char *path) {
   int applied = 0;
   while (fgets(line, sizeof line, f)) {
     int op, K, L;
-    unsigned prec;
+    unsigned prec, maxc = 0;
     if (line[0] == '#') continue;
-    if (sscanf(line, "%d %u %d %d", &op, &prec, &K, &L) != 4) continue;
+    int got = sscanf(line, "%d %u %d %d %u", &op, &prec, &K, &L, &maxc);
+    if (got != 4 && got != 5) continue;
     int nv = fpa_kernel_variants(op, prec);
     for (int v = 0; v < nv; v++) {
       int k, l;
       fpa_kernel_variant_shape(op, prec, v, &k, &l);
-      if (k == K && l == L) { fpa_kernel_set_variant(op, prec, v); applied++; break; }
+      if (k != K || l != L) continue;
+      if (got == 5 && maxc > 0) fpa_kernel_set_small_variant(op, prec, v, maxc);
+      else set_large_variant(op, prec, v);
+      applied++;
+      break;
     }
   }
   fclose(f);
@@ -253,7 +285,7 @@ static fpa_result run(int op, fpa_ptr r, fpa_ptr a, fpa_ptr b, fpa_rounding_mode
   args.b_exps = b ? b->gpu_exps : a->gpu_exps;
   args.b_limbs = b ? b->gpu_limbs : a->gpu_limbs;
   if (count == 0) return fpa_success;
-  int rc = fpa_kernel_launch(op, &args, variant_tab[op][p], cur_stream);
+  int rc = fpa_kernel_launch(op, &args, fpa_kernel_get_variant_for_count(op, precision, count), cur_stream);
   if (rc != 0) {
     fprintf(stderr, "\nkernel launch for op %d, precision %u failed: %d (%s)\n", op, precision, rc,
             rc > 0 ? hipGetErrorString((hipError_t)rc) : "no such variant");
@@ -339,8 +371,8 @@ fpa_result fpa_set_double(fpa_t r, double *doubles) {
 
 /* set / sgn / cmp (fpa_dispatch.h:88-153): the kernels in k_setcmp.hip, launched on the
  * fpa stream with the dispatcher's neg shape (a copy-like kernel) for the precision. */
-static void copy_shape(uint32_t precision, int *K, int *L) {
-  fpa_kernel_variant_shape(FPA_OP_NEG, precision, fpa_kernel_get_variant(FPA_OP_NEG, precision), K, L);
+static void copy_shape(uint32_t precision, uint32_t count, int *K, int *L) {
+  fpa_kernel_variant_shape(FPA_OP_NEG, precision, fpa_kernel_get_variant_for_count(FPA_OP_NEG, precision, count), K, L);
 }
 
 fpa_result fpa_set(fpa_t r, fpa_t x, fpa_rounding_mode mode) {
@@ -359,7 +391,7 @@ fpa_result fpa_set(fpa_t r, fpa_t x, fpa_rounding_mode mode) {
   a.a_exps = x->gpu_exps;
   a.a_limbs = x->gpu_limbs;
   int K, L;
-  copy_shape(r->precision, &K, &L);
+  copy_shape(r->precision, r->count, &K, &L);
   int rc = fpa_impl_launch_setcmp(1, &a, K, L, cur_stream);
   if (rc != 0) { fprintf(stderr, "fpa_set: launch failed (%d)\n", rc); exit(1); }
   return fpa_success;
@@ -385,7 +417,7 @@ fpa_result fpa_cmp(int32_t *r, fpa_t a, fpa_t b) {
   g.b_exps = b->gpu_exps;
   g.b_limbs = b->gpu_limbs;
   int K, L;
-  copy_shape(a->precision, &K, &L);
+  copy_shape(a->precision, a->count, &K, &L);
   int rc = fpa_impl_launch_setcmp(0, &g, K, L, cur_stream);
   if (rc != 0) { fprintf(stderr, "fpa_cmp: launch failed (%d)\n", rc); exit(1); }
   return fpa_success;

@@ -169,7 +169,7 @@ static void launch_groups(const fpa_request *q, const layout_t *L, uint8_t *dev,
     a.a_limbs = (uint32_t *)(dev + g->a_l);
     a.b_exps = (uint32_t *)(dev + g->b_e);
     a.b_limbs = (uint32_t *)(dev + g->b_l);
-    int rc = fpa_kernel_launch(g->op, &a, fpa_kernel_get_variant(g->op, q->precision), st);
+    int rc = fpa_kernel_launch(g->op, &a, fpa_kernel_get_variant_for_count(g->op, q->precision, g->count), st);
     if (rc != 0) { fprintf(stderr, "server: launch failed (%d)\n", rc); exit(1); }
   }
 }

@@ -60,12 +60,20 @@ int fpa_kernel_variants(int op, uint32_t precision);
 /* Shape of a variant: limbs per lane, lanes per value. Returns 0, or -1 if invalid. */
 int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *limbs_per_lane, int *lanes_per_value);
 
-/* Variant the fpa.h dispatcher uses for (op, precision), and an override of it
- * (the autotuner writes these).  Returns -1 for invalid arguments. */
+/* Variant the fpa.h dispatcher uses for (op, precision), and an override of it (the
+ * autotuner writes these; an override applies to every launch size, so it also removes
+ * the small-launch variant below).  Returns -1 for invalid arguments. */
 int fpa_kernel_get_variant(int op, uint32_t precision);
 int fpa_kernel_set_variant(int op, uint32_t precision, int variant);
 
-/* Load "op precision K L" lines (as written by the autotuner) into the dispatch table.
+/* Small launches: a second variant used for launches of at most max_count values
+ * (max_count 0 removes it), and the variant the dispatcher uses for a launch of `count`
+ * values (the fpa.h calls, batch segments, server groups and dot chains all ask this). */
+int fpa_kernel_set_small_variant(int op, uint32_t precision, int variant, uint32_t max_count);
+int fpa_kernel_get_variant_for_count(int op, uint32_t precision, uint64_t count);
+
+/* Load "op precision K L" lines (as written by the autotuner) into the dispatch table;
+ * a fifth field max_count makes the line a small-launch variant.
  * Returns the number of entries applied, or -1 if the file cannot be read.
  * When the library is loaded it applies, if present, the file named by the environment
  * variable FPA_TUNING_FILE, else tuning.txt in the directory holding the library

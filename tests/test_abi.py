@@ -110,9 +110,31 @@ def test_load_time_tuning_matches_committed_table():
     lines = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
     if os.environ.get("FPA_TUNING_FILE") is None:
         assert F.lib.fpa_kernel_tuning_applied() == len(lines)
-    for op, prec, k, l in lines:
-        name = {v: n for n, v in F.KERNELS.items()}[int(op)]
-        assert (int(k), int(l)) in F.variants(name, int(prec))
+    for f in lines:
+        assert len(f) in (4, 5)
+        op, prec, k, l = (int(x) for x in f[:4])
+        name = {v: n for n, v in F.KERNELS.items()}[op]
+        assert (k, l) in F.variants(name, prec)
+
+
+def test_small_launch_variants():
+    keep = F.get_variant("div", 4096), F.get_variant_for_count("div", 4096, 100)
+    nv = len(F.variants("div", 4096))
+    try:
+        F.set_variant("div", 4096, 0)
+        assert F.get_variant_for_count("div", 4096, 100) == 0
+        assert F.set_small_variant("div", 4096, nv - 1, 5000) == 0
+        assert F.get_variant_for_count("div", 4096, 5000) == nv - 1
+        assert F.get_variant_for_count("div", 4096, 5001) == 0
+        assert F.get_variant("div", 4096) == 0
+        assert F.set_small_variant("div", 4096, nv, 5000) == -1      # no such variant
+        # an explicit override applies to every launch size
+        assert F.set_variant("div", 4096, 1) == 0
+        assert F.get_variant_for_count("div", 4096, 10) == 1
+    finally:
+        F.set_variant("div", 4096, keep[0])
+        if keep[1] != keep[0]:
+            F.set_small_variant("div", 4096, keep[1], 16384)
 
 
 def test_scalar_helpers_vs_python():
@@ -173,11 +195,14 @@ def test_autotune_table_round_trip(tmp_path):
         pick = vs[-1]
         table[(op, p)] = (pick, [(k, l, 1.0 + i) for i, (k, l) in enumerate(vs)], 1234)
     f = tmp_path / "t.txt"
-    T.write_table(str(f), table)
+    small = {("div", 4096): (F.variants("div", 4096)[0], [(k, l, 1.0) for k, l in F.variants("div", 4096)], 777)}
+    T.write_table(str(f), table, small)
     try:
-        assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == len(table)
+        assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == len(table) + 1
         for (op, p) in keep:
             assert F.get_variant(op, p) == len(F.variants(op, p)) - 1
+        assert F.get_variant_for_count("div", 4096, 777) == 0
+        assert F.get_variant_for_count("div", 4096, 778) == len(F.variants("div", 4096)) - 1
     finally:
         for (op, p), v in keep.items():
             F.set_variant(op, p, v)
