@@ -308,12 +308,13 @@ __host__ __device__ constexpr int mul_digits_per_lane(int K) { return (32 * K + 
 /* digit chunks in LDS are padded to whole 16-byte rows */
 __host__ __device__ constexpr int mul_digits_padded(int K) { return (mul_digits_per_lane(K) + 3) / 4 * 4; }
 /* groups of 2..16 lanes sit inside one 16-lane DPP row: their window shift is a DPP
- * row shift and their broadcasts come from LDS */
+ * row shift; groups of 32 shift through ds_bpermute */
 __host__ __device__ constexpr bool mul_row_path(int L) { return L >= 2 && L <= 16; }
-/* LDS words per value: 64-bit low columns (later the digits), plus a's digit chunks on
- * the row path */
+/* groups of 2 or more lanes take a's digit chunk of each round from LDS */
+__host__ __device__ constexpr bool mul_lds_broadcast(int L) { return L >= 2; }
+/* LDS words per value: 64-bit low columns (later the digits), plus a's digit chunks */
 __host__ __device__ constexpr int mul_lds_words(int K, int L) {
-  return 2 * L * mul_digits_per_lane(K) + (mul_row_path(L) ? L * mul_digits_padded(K) : 0);
+  return 2 * L * mul_digits_per_lane(K) + (mul_lds_broadcast(L) ? L * mul_digits_padded(K) : 0);
 }
 
 /* x of lane+1 within this lane's 16-lane row (DPP row_shl:1); the row's lane 15 gets 0 */
@@ -427,7 +428,7 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
 #pragma unroll
   for (int i = 0; i < 2 * KD; i++) W[i] = 0;
   uint64_t *low = reinterpret_cast<uint64_t *>(lds);   /* ND 64-bit low column sums */
-  if constexpr (mul_row_path(L)) {
+  if constexpr (mul_lds_broadcast(L)) {
     uint32_t *adig = lds + 2 * ND;                      /* a's digit chunks, KDP apart */
 #pragma unroll
     for (int k = 0; k < KD; k++) adig[g.gl * KDP + k] = A[k];
@@ -448,48 +449,48 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
 #pragma unroll
         for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
       }
-      /* Lockstep: a group's top lane reads lane 0 of the next group in its row (not
-       * zero); that value is the next group's complete chunk r, and it ends up, unchanged,
-       * in lane r's final window, where it is subtracted below. */
-      const bool top = !kLockstep && g.gl == L - 1;
+      if constexpr (mul_row_path(L)) {
+        /* Lockstep: a group's top lane reads lane 0 of the next group in its row (not
+         * zero); that value is the next group's complete chunk r, and it ends up,
+         * unchanged, in lane r's final window, where it is subtracted below. */
+        const bool top = !kLockstep && g.gl == L - 1;
 #pragma unroll
-      for (int i = 0; i < KD; i++) {
-        uint32_t lo = dpp_row_next((uint32_t)W[i]);
-        uint32_t hi = dpp_row_next((uint32_t)(W[i] >> 32));
-        if (!kLockstep) {
-          lo = top ? 0u : lo;
-          hi = top ? 0u : hi;
+        for (int i = 0; i < KD; i++) {
+          uint32_t lo = dpp_row_next((uint32_t)W[i]);
+          uint32_t hi = dpp_row_next((uint32_t)(W[i] >> 32));
+          if (!kLockstep) {
+            lo = top ? 0u : lo;
+            hi = top ? 0u : hi;
+          }
+          /* one add-with-carry per half, so each row shift can fold into its add */
+          unsigned int c, c2;
+          const uint32_t nlo = __builtin_addc((uint32_t)W[KD + i], lo, 0u, &c);
+          const uint32_t nhi = __builtin_addc((uint32_t)(W[KD + i] >> 32), hi, c, &c2);
+          W[i] = ((uint64_t)nhi << 32) | nlo;
+          W[KD + i] = 0;
         }
-        /* one add-with-carry per half, so each row shift can fold into its add */
-        unsigned int c, c2;
-        const uint32_t nlo = __builtin_addc((uint32_t)W[KD + i], lo, 0u, &c);
-        const uint32_t nhi = __builtin_addc((uint32_t)(W[KD + i] >> 32), hi, c, &c2);
-        W[i] = ((uint64_t)nhi << 32) | nlo;
-        W[KD + i] = 0;
+      } else {
+        /* the group spans DPP rows: next window = own high half + next lane's low half */
+#pragma unroll
+        for (int i = 0; i < KD; i++) {
+          uint32_t lo = g.from_next((uint32_t)W[i]);
+          uint32_t hi = g.from_next((uint32_t)(W[i] >> 32));
+          W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
+          W[KD + i] = 0;
+        }
       }
     }
   } else {
-    for (int r = 0; r < L; r++) {
+    /* one lane per value: the whole product is in this lane's window */
 #pragma unroll
-      for (int i = 0; i < KD; i++) {
-        const uint32_t ai = g.bcast(A[i], r);
+    for (int i = 0; i < KD; i++) {
 #pragma unroll
-        for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ai * B[j];
-      }
-      /* group lane 0's low window: columns r*KD .. r*KD+KD-1 are complete */
-      if (g.gl == 0) {
-#pragma unroll
-        for (int i = 0; i < KD; i++) low[r * KD + i] = W[i];
-      }
-      /* next window = own high half + next lane's low half */
-#pragma unroll
-      for (int i = 0; i < KD; i++) {
-        uint32_t lo = g.from_next((uint32_t)W[i]);
-        uint32_t hi = g.from_next((uint32_t)(W[i] >> 32));
-        W[i] = W[KD + i] + (((uint64_t)hi << 32) | lo);
-        W[KD + i] = 0;
-      }
+      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)A[i] * B[j];
     }
+#pragma unroll
+    for (int i = 0; i < KD; i++) low[i] = W[i];
+#pragma unroll
+    for (int i = 0; i < KD; i++) { W[i] = W[KD + i]; W[KD + i] = 0; }
   }
   wsync();
   if constexpr (kLockstep && mul_row_path(L) && L < 16) {
