@@ -71,12 +71,11 @@ static double wave_cost(int op, int K, int L) {
   }
 }
 
-static const seg_host *sort_base;
+typedef struct { double cost; uint32_t seg; } seg_cost;
 static int seg_cmp(const void *x, const void *y) {
-  uint32_t i = *(const uint32_t *)x, j = *(const uint32_t *)y;
-  double a = sort_base[i].cost, b = sort_base[j].cost;
-  if (a != b) return a > b ? -1 : 1;
-  return i < j ? -1 : (i > j);
+  const seg_cost *a = (const seg_cost *)x, *b = (const seg_cost *)y;
+  if (a->cost != b->cost) return a->cost > b->cost ? -1 : 1;
+  return a->seg < b->seg ? -1 : (a->seg > b->seg);
 }
 
 static int launch_segment(const seg_host *g, void *stream) {
@@ -152,9 +151,11 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
   }
   /* longest processing time first onto the least-loaded branch */
   b->order = (uint32_t *)malloc(sizeof(uint32_t) * (b->nseg ? b->nseg : 1));
-  for (uint32_t s = 0; s < b->nseg; s++) b->order[s] = s;
-  sort_base = b->sh;
-  qsort(b->order, b->nseg, sizeof(uint32_t), seg_cmp);
+  seg_cost *sc = (seg_cost *)malloc(sizeof(seg_cost) * (b->nseg ? b->nseg : 1));
+  for (uint32_t s = 0; s < b->nseg; s++) { sc[s].cost = b->sh[s].cost; sc[s].seg = s; }
+  qsort(sc, b->nseg, sizeof(seg_cost), seg_cmp);
+  for (uint32_t s = 0; s < b->nseg; s++) b->order[s] = sc[s].seg;
+  free(sc);
   b->branches = branches < b->nseg ? branches : (b->nseg ? b->nseg : 1);
   double load[FPA_BATCH_MAX_BRANCHES] = { 0 };
   for (uint32_t k = 0; k < b->nseg; k++) {

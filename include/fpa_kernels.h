@@ -66,6 +66,8 @@ int fpa_kernel_variant_shape(int op, uint32_t precision, int variant, int *limbs
 int fpa_kernel_get_variant(int op, uint32_t precision);
 int fpa_kernel_set_variant(int op, uint32_t precision, int variant);
 
+/* The variant tables are process-wide: set them before launching from several threads. */
+
 /* Small launches: a second variant used for launches of at most max_count values
  * (max_count 0 removes it), and the variant the dispatcher uses for a launch of `count`
  * values (the fpa.h calls, batch segments, server groups and dot chains all ask this). */
