@@ -312,9 +312,18 @@ __host__ __device__ constexpr int mul_digits_padded(int K) { return (mul_digits_
 __host__ __device__ constexpr bool mul_row_path(int L) { return L >= 2 && L <= 16; }
 /* groups of 2 or more lanes take a's digit chunk of each round from LDS */
 __host__ __device__ constexpr bool mul_lds_broadcast(int L) { return L >= 2; }
-/* LDS words per value: 64-bit low columns (later the digits), plus a's digit chunks */
+/* word offset of a's digit chunks: the first 16-byte row at or above ND = L*KD */
+__host__ __device__ constexpr int mul_adig_offset(int K, int L) { return (L * mul_digits_per_lane(K) + 3) / 4 * 4; }
+/* LDS words per value: 2*ND words of 64-bit low columns (later the digits).  a's digit
+ * chunks overlap their top half (chunk r at mul_adig_offset + r*KDP): lane 0 fills the
+ * low columns from the bottom, 2*KD words per round, and after round r has written
+ * 2*(r+1)*KD <= ND + (r+1)*KDP words (KDP >= KD), so chunks still to be read are never
+ * overwritten. */
 __host__ __device__ constexpr int mul_lds_words(int K, int L) {
-  return 2 * L * mul_digits_per_lane(K) + (mul_lds_broadcast(L) ? L * mul_digits_padded(K) : 0);
+  return !mul_lds_broadcast(L) ? 2 * L * mul_digits_per_lane(K)
+         : 2 * L * mul_digits_per_lane(K) > mul_adig_offset(K, L) + L * mul_digits_padded(K)
+             ? 2 * L * mul_digits_per_lane(K)
+             : mul_adig_offset(K, L) + L * mul_digits_padded(K);
 }
 
 /* x of lane+1 within this lane's 16-lane row (DPP row_shl:1); the row's lane 15 gets 0 */
@@ -385,7 +394,7 @@ __device__ __forceinline__ uint32_t add_carry_digits(uint32_t (&dg)[KD], uint64_
 }
 
 /* (re, R) = a * b; lds: mul_lds_words(K, L) words (64-bit low columns, then digits; a's
- * digit chunks after them on the row path).  group_stride = LDS words from this group's
+ * digit chunks in their top half while the rounds run).  group_stride = LDS words from this group's
  * base to the next group's in the same wave.
  * kLockstep: every group of the wave calls this together (the element-wise kernel);
  * special-class operands then run the digit rounds too and take their class result at
@@ -429,7 +438,7 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
   for (int i = 0; i < 2 * KD; i++) W[i] = 0;
   uint64_t *low = reinterpret_cast<uint64_t *>(lds);   /* ND 64-bit low column sums */
   if constexpr (mul_lds_broadcast(L)) {
-    uint32_t *adig = lds + 2 * ND;                      /* a's digit chunks, KDP apart */
+    uint32_t *adig = lds + mul_adig_offset(K, L);       /* a's digit chunks, KDP apart */
 #pragma unroll
     for (int k = 0; k < KD; k++) adig[g.gl * KDP + k] = A[k];
     wsync();
@@ -464,8 +473,8 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
           }
           /* one add-with-carry per half, so each row shift can fold into its add */
           unsigned int c, c2;
-          const uint32_t nlo = __builtin_addc((uint32_t)W[KD + i], lo, 0u, &c);
-          const uint32_t nhi = __builtin_addc((uint32_t)(W[KD + i] >> 32), hi, c, &c2);
+          const uint32_t nlo = __builtin_addc(lo, (uint32_t)W[KD + i], 0u, &c);
+          const uint32_t nhi = __builtin_addc(hi, (uint32_t)(W[KD + i] >> 32), c, &c2);
           W[i] = ((uint64_t)nhi << 32) | nlo;
           W[KD + i] = 0;
         }
