@@ -20,6 +20,10 @@
   X(256, 16, 16) X(256, 32, 8)  X(256, 8, 32)                \
   X(512, 16, 32) X(512, 32, 16)
 
+/* Matrix-core mul kernels (op mul only): (n, K, L) with one value per wave, L = 64 lanes
+ * holding K limbs each on input; the 4096-bit kernel computes the product with int8 MFMAs. */
+#define FPA_MFMA_SHAPES(X) X(128, 2, 64)
+
 /* The dot-chain kernel (one mul core and one add core per element) is compiled only for
  * shapes with K <= FPA_DOT_MAX_K: above that its add core no longer inlines and becomes a
  * real call, which no kernel of this library may contain. */
