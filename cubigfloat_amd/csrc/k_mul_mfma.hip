@@ -16,10 +16,12 @@
  * each lane reads its row as four aligned dwords.  B_{d+1} is B_d moved up one lane
  * (DPP wave_shr:1): lane 0 receives 0, and lane 32 receives lane 31's chunk 31-d, which
  * is past a's top for every d <= 15.
- * Then a*b = sum_k c_k 256^k + X*(ca*b + cb*a)  (mod X^2, and 0 <= a*b < X^2).  Column
- * groups of four go through LDS so that lane L owns product limbs 4L..4L+3; signed carries
- * between lanes are resolved by the ballot lookahead.  Lanes 32..63 then hold the top
- * half and round it as a 32-lane group with 4 limbs per lane.
+ * Then a*b = sum_k c_k 256^k + X*(ca*b_s + cb*a_s)  (mod X^2, and 0 <= a*b < X^2): the
+ * high-half term is signed digits too, added by two more MFMAs with an identity T (a's
+ * chunks at columns 16..31 are B_16 itself; b's come from LDS).  Column groups of four go
+ * through LDS so that lane L owns product limbs 4L..4L+3; signed carries between lanes
+ * are resolved by the ballot lookahead.  Lanes 32..63 then hold the top half and round
+ * it as a 32-lane group with 4 limbs per lane.
  */
 #include "fpa_kernel_common.h"
 
@@ -28,12 +30,14 @@ namespace fpa {
 typedef int mm_v4 __attribute__((ext_vector_type(4)));
 typedef int mm_v16 __attribute__((ext_vector_type(16)));
 
-/* per-wave LDS, bytes: four reversed copies of b's digits, then a's digits.  A copy holds
- * 144 dwords (dwords 8..135 = b reversed, zeros around) in a 168-dword slot, which puts
- * the four copies 8 banks apart.  The column stage (256 x 8 bytes) reuses [0, 2048). */
+/* per-wave LDS, bytes: four reversed copies of b's digits, then a's and b's digits.  A
+ * copy holds 144 dwords (dwords 8..135 = b reversed, zeros around) in a 168-dword slot,
+ * which puts the four copies 8 banks apart.  The column stage (256 x 8 bytes, element w
+ * at w + w/16) reuses [0, 2176). */
 constexpr int MM_TS = 672;
 constexpr int MM_A = 4 * MM_TS;
-constexpr int MM_WAVE = MM_A + 512;
+constexpr int MM_B = MM_A + 512;
+constexpr int MM_WAVE = MM_B + 512;
 
 __device__ __forceinline__ uint32_t mm_wave_shr1(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xF, 0xF, true);
@@ -56,14 +60,9 @@ __global__ __launch_bounds__(256) void fpa_mul_mfma_kernel(fpa_arguments a) {
     return;
   }
   const Group<64> g;
-  const uint32_t *ap = a.a_limbs + (size_t)v * 128, *bp = a.b_limbs + (size_t)v * 128;
-  uint32_t A[2], B[2], AH[4] = {0, 0, 0, 0}, BH[4] = {0, 0, 0, 0};
-  load_chunk(A, ap + 2 * lane);
-  load_chunk(B, bp + 2 * lane);
-  if (lane >= 32) {                    /* limbs 4(L-32).. for the X*(ca*b + cb*a) term */
-    load_chunk(AH, ap + 4 * (lane - 32));
-    load_chunk(BH, bp + 4 * (lane - 32));
-  }
+  uint32_t A[2], B[2];
+  load_chunk(A, a.a_limbs + (size_t)v * 128 + 2 * lane);
+  load_chunk(B, a.b_limbs + (size_t)v * 128 + 2 * lane);
 
   /* signed digits: t = x + 0x8080..80 */
   const uint32_t KC[2] = {0x80808080u, 0x80808080u};
@@ -74,6 +73,7 @@ __global__ __launch_bounds__(256) void fpa_mul_mfma_kernel(fpa_arguments a) {
   /* LDS: a's digits; reversed b digits R (R dword 135-w = bswap(b digit dword w)) in four
    * copies R_s[z] = R[z + s]; dwords 0..7 and 136..143 of R are zero */
   *reinterpret_cast<uint2 *>(ws + MM_A + 8 * lane) = make_uint2(TA[0] ^ 0x80808080u, TA[1] ^ 0x80808080u);
+  *reinterpret_cast<uint2 *>(ws + MM_B + 8 * lane) = make_uint2(TB[0] ^ 0x80808080u, TB[1] ^ 0x80808080u);
   {
     const int sc = lane >> 4, k = lane & 15;
     *reinterpret_cast<uint32_t *>(ws + sc * MM_TS + 4 * (k < 8 ? k : 128 + k)) = 0u;
@@ -91,10 +91,11 @@ __global__ __launch_bounds__(256) void fpa_mul_mfma_kernel(fpa_arguments a) {
 
   /* 17 MFMAs: lane (h, r) holds row r of T_d and column r of B_d, bytes 16h..16h+15 of k */
   const int h = lane >> 5, r = lane & 31;
-  mm_v4 bb = {0, 0, 0, 0};
+  mm_v4 bb = {0, 0, 0, 0}, bs = {0, 0, 0, 0};
   if (r < 16) bb = *reinterpret_cast<const mm_v4 *>(ws + MM_A + 32 * r + 16 * h);
-  const int sl = (3 - r) & 3;                                     /* byte shift of this row */
-  int qb = (543 - r + 16 * h - sl) / 4 - 128 + sl * (MM_TS / 4);  /* dwords, >= 0 */
+  else bs = *reinterpret_cast<const mm_v4 *>(ws + MM_B + 32 * (r - 16) + 16 * h);   /* b's chunk r-16 */
+  const uint32_t sl = (3u - (uint32_t)r) & 3u;                     /* byte shift of this row */
+  int qb = (int)((543u - (uint32_t)r + 16u * (uint32_t)h - sl) >> 2) - 128 + (int)sl * (MM_TS / 4);   /* >= 0 */
   asm volatile("" : "+v"(qb));         /* one base register; the 17 rows are immediate offsets */
   const uint32_t *tb = reinterpret_cast<const uint32_t *>(ws) + qb;
   mm_v16 acc = {};
@@ -114,28 +115,47 @@ __global__ __launch_bounds__(256) void fpa_mul_mfma_kernel(fpa_arguments a) {
     }
   }
 
+  /* high half: + cb*a_s and + ca*b_s at columns 512.. (T = identity: row m has a 1 at k = m) */
+  {
+    const int j = r - 16 * h;          /* byte of this lane's 16 that is the diagonal */
+    mm_v4 ti = {0, 0, 0, 0};
+    if (j >= 0 && j < 16) {
+      const int one = 1 << (8 * (j & 3));
+      ti.x = (j >> 2) == 0 ? one : 0;
+      ti.y = (j >> 2) == 1 ? one : 0;
+      ti.z = (j >> 2) == 2 ? one : 0;
+      ti.w = (j >> 2) == 3 ? one : 0;
+    }
+    if (cb) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(ti, bb, acc, 0, 0, 0);   /* bb = B_16 */
+    if (ca) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(ti, bs, acc, 0, 0, 0);
+  }
+
   /* acc[4q+i] = column 32r + 8q + 4h + i: dword w = 8r + 2q + h of the product gets
-   * V_w = sum_i acc[4q+i] 2^(8i) (signed, |V_w| < 2^48) */
+   * V_w = sum_i acc[4q+i] 2^(8i) (signed, |V_w| < 2^48), as three 32x32->64 signed mads */
   int64_t *V = reinterpret_cast<int64_t *>(ws);
+  int w8 = 1 << 8, w16 = 1 << 16, w24 = 1 << 24;
+  asm volatile("" : "+s"(w8), "+s"(w16), "+s"(w24));   /* multiplies, not 64-bit shifts */
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int64_t vq = (int64_t)acc[4 * q] + ((int64_t)acc[4 * q + 1] << 8) + ((int64_t)acc[4 * q + 2] << 16) +
-                       ((int64_t)acc[4 * q + 3] << 24);
-    V[8 * r + 2 * q + h] = vq;
+    int64_t vq = (int64_t)acc[4 * q];
+    vq = (int64_t)acc[4 * q + 1] * (int64_t)w8 + vq;
+    vq = (int64_t)acc[4 * q + 2] * (int64_t)w16 + vq;
+    vq = (int64_t)acc[4 * q + 3] * (int64_t)w24 + vq;
+    V[8 * r + 2 * q + h + (r >> 1)] = vq;   /* element w at w + w/16 */
   }
   /* lane L: product limbs 4L..4L+3 = low word of V_w plus the high part of V_(w-1) */
-  int64_t vw[5];
-  vw[0] = lane ? V[4 * lane - 1] : 0;
-#pragma unroll
-  for (int t = 0; t < 4; t++) vw[t + 1] = V[4 * lane + t];
+  const int base = 4 * lane + (lane >> 2);
+  int32_t hprev = 0;
+  if (lane) hprev = reinterpret_cast<const int32_t *>(V)[2 * (4 * lane - 1 + ((lane - 1) >> 2)) + 1];
   uint32_t D[4];
   int64_t c = 0;
 #pragma unroll
   for (int t = 0; t < 4; t++) {
-    const int64_t sum = (int64_t)(uint32_t)vw[t + 1] + (vw[t] >> 32) + (ca ? (int64_t)BH[t] : 0) +
-                        (cb ? (int64_t)AH[t] : 0) + c;
+    const int64_t vt = V[base + t];
+    const int64_t sum = (int64_t)(uint32_t)vt + (int64_t)hprev + c;
     D[t] = (uint32_t)sum;
-    c = sum >> 32;                     /* in [-1, 3] */
+    c = sum >> 32;                     /* in [-1, 1] */
+    hprev = (int32_t)(vt >> 32);
   }
   /* signed carry from the lane below, then the 1-bit carries/borrows it leaves; the carry
    * out of limb 255 is the X^2 term and is dropped */
