@@ -1,10 +1,11 @@
-"""The 4096-bit matrix-core mul kernel (shape K=2, L=64: one value per wave, int8 MFMAs).
+"""The matrix-core mul kernels (shapes (n/64, 64): one value per wave, int8 MFMAs) at
+2048, 4096, 6144, 8192 and 16384 bits.
 
 Bar: bit-exact.  Against the oracle on config-2 operands in every rounding mode, on the
 reference generator's adversarial limb alphabet, on all 9x9 class pairs and at the TINY/INF
 exponent thresholds; against the radix-2^27 kernel (the default shape) on raw random limbs,
 including mantissas whose top bit is clear (the recode carry is then 0), all-zero and
-all-ones mantissas, and on a launch of 2^16 values.
+all-ones mantissas, and on a larger launch.
 """
 import sys
 
@@ -16,8 +17,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-P = 4096
-N = P // 32
+MFMA_PRECISIONS = [2048, 4096, 6144, 8192, 16384]
 
 
 def say(*a):
@@ -25,76 +25,84 @@ def say(*a):
     sys.stdout.flush()
 
 
-def mfma_variant():
-    shapes = F.variants("mul", P)
-    assert (2, 64) in shapes, shapes
-    return shapes.index((2, 64))
+def mfma_variant(p):
+    shapes = F.variants("mul", p)
+    assert (p // 2048, 64) in shapes, shapes
+    return shapes.index((p // 2048, 64))
 
 
-def run_variant(v, ae, al, be, bl, mode="rne"):
-    default = F.get_variant("mul", P)
+def run_variant(p, v, ae, al, be, bl, mode="rne"):
+    default = F.get_variant("mul", p)
     try:
-        assert F.set_variant("mul", P, v) == 0
-        return F.run_op("mul", P, ae, al, be, bl, mode=F.MODES[mode])
+        assert F.set_variant("mul", p, v) == 0
+        return F.run_op("mul", p, ae, al, be, bl, mode=F.MODES[mode])
     finally:
-        F.set_variant("mul", P, default)
+        F.set_variant("mul", p, default)
 
 
-def check_oracle(ae, al, be, bl, mode, tag):
-    say(f"[gpu] mul mfma p={P} count={len(ae)} mode={mode} {tag}")
-    re, rl = run_variant(mfma_variant(), ae, al, be, bl, mode)
-    oe, ol = O.op("mul", P, ae, al, be, bl, mode=mode)
-    eq = O.results_equal(re, rl, oe, ol, P)
-    assert eq.all(), f"{tag} {mode}: {int((~eq).sum())} mismatches, first {int(np.where(~eq)[0][0])}"
+def check_oracle(p, ae, al, be, bl, mode, tag):
+    say(f"[gpu] mul mfma p={p} count={len(ae)} mode={mode} {tag}")
+    re, rl = run_variant(p, mfma_variant(p), ae, al, be, bl, mode)
+    oe, ol = O.op("mul", p, ae, al, be, bl, mode=mode)
+    eq = O.results_equal(re, rl, oe, ol, p)
+    assert eq.all(), f"p={p} {tag} {mode}: {int((~eq).sum())} mismatches, first {int(np.where(~eq)[0][0])}"
 
 
-def check_valu(ae, al, be, bl, mode, tag):
-    say(f"[gpu] mul mfma vs radix-2^27 p={P} count={len(ae)} mode={mode} {tag}")
-    v = mfma_variant()
-    valu = next(i for i, (k, l) in enumerate(F.variants("mul", P)) if l != 64)
-    re, rl = run_variant(v, ae, al, be, bl, mode)
-    we, wl = run_variant(valu, ae, al, be, bl, mode)
-    assert np.array_equal(re, we), f"{tag}: exponent words differ at {int(np.where(re != we)[0][0])}"
-    assert np.array_equal(rl, wl), f"{tag}: limbs differ in value {int(np.where(rl != wl)[0][0]) // N}"
+def check_valu(p, ae, al, be, bl, mode, tag):
+    say(f"[gpu] mul mfma vs radix-2^27 p={p} count={len(ae)} mode={mode} {tag}")
+    valu = next(i for i, (k, l) in enumerate(F.variants("mul", p)) if l != 64)
+    re, rl = run_variant(p, mfma_variant(p), ae, al, be, bl, mode)
+    we, wl = run_variant(p, valu, ae, al, be, bl, mode)
+    assert np.array_equal(re, we), f"p={p} {tag}: exponent words differ at {int(np.where(re != we)[0][0])}"
+    assert np.array_equal(rl, wl), f"p={p} {tag}: limbs differ in value {int(np.where(rl != wl)[0][0]) // (p // 32)}"
 
 
-def test_variant_is_compiled_for_mul_4096_only():
-    assert (2, 64) in F.variants("mul", P)
-    for op in ("add", "sub", "div", "sqrt", "dot"):
-        assert (2, 64) not in F.variants(op, P), op
+def count(p, base):
+    return max(16, base * 4096 // p) if p >= 4096 else base
+
+
+def test_variants_are_compiled_for_mul_only():
     for p in F.PRECISIONS:
-        if p != P:
-            assert (2, 64) not in F.variants("mul", p), p
+        has = any(l == 64 for _, l in F.variants("mul", p))
+        assert has == (p in MFMA_PRECISIONS), p
+        for op in ("add", "sub", "div", "sqrt", "dot"):
+            assert all(l != 64 for _, l in F.variants(op, p)), (op, p)
 
 
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
 @pytest.mark.parametrize("mode", ["rne", "rz", "ru", "rd", "rna"])
-def test_config2_all_modes(mode):
-    ae, al, be, bl = F.generate(2, 311, "mul", P, 2048)
-    check_oracle(ae, al, be, bl, mode, "config-2")
+def test_config2_all_modes(p, mode):
+    ae, al, be, bl = F.generate(2, 311, "mul", p, count(p, 2048))
+    check_oracle(p, ae, al, be, bl, mode, "config-2")
 
 
-def test_adversarial_alphabet():
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
+def test_adversarial_alphabet(p):
     """Limbs from {0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF} (tests.c:173-181), which
     make long runs of equal digits and carries through whole chunks."""
-    rng = np.random.default_rng(17)
-    cnt = 512
+    rng = np.random.default_rng(17 + p)
+    cnt = count(p, 512)
+    N = p // 32
     alph = np.array([0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF], np.uint32)
     al = alph[rng.integers(0, 5, size=(cnt, N))]
     bl = alph[rng.integers(0, 5, size=(cnt, N))]
     al[:, -1] |= np.uint32(0x80000000)
     bl[:, -1] |= np.uint32(0x80000000)
-    al[:64] = 0xFFFFFFFF                      # all-ones mantissas: the largest product
-    bl[:32] = 0xFFFFFFFF
-    al[64:96] = 0
-    al[64:96, -1] = 0x80000000                # 2^-1 exactly: the product is b, shifted
+    q = cnt // 8
+    al[:2 * q] = 0xFFFFFFFF                   # all-ones mantissas: the largest product
+    bl[:q] = 0xFFFFFFFF
+    al[2 * q:3 * q] = 0
+    al[2 * q:3 * q, -1] = 0x80000000          # 2^-1 exactly: the product is b, shifted
     e = O.BIAS + rng.integers(-40, 40, size=cnt)
     ae = (2 * e + rng.integers(0, 2, cnt)).astype(np.uint32)
     be = (2 * (O.BIAS + rng.integers(-40, 40, size=cnt)) + rng.integers(0, 2, cnt)).astype(np.uint32)
     for mode in ("rne", "rz", "ru", "rd", "rna"):
-        check_oracle(ae, al.reshape(-1), be, bl.reshape(-1), mode, "adversarial")
+        check_oracle(p, ae, al.reshape(-1), be, bl.reshape(-1), mode, "adversarial")
 
 
-def test_special_classes():
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
+def test_special_classes(p):
+    N = p // 32
     codes = [2 * (O.BIAS + 3), 2 * (O.BIAS + 3) + 1, 2, 3, 4, 5, 6, 7, 8]
     ae = np.repeat(np.array(codes, np.uint32), 9)
     be = np.tile(np.array(codes, np.uint32), 9)
@@ -103,12 +111,14 @@ def test_special_classes():
     bl = rng.integers(0, 2**32, size=81 * N, dtype=np.uint64).astype(np.uint32)
     al.reshape(81, N)[:, -1] |= np.uint32(0x80000000)
     bl.reshape(81, N)[:, -1] |= np.uint32(0x80000000)
-    check_oracle(ae, al, be, bl, "rne", "classes")
+    check_oracle(p, ae, al, be, bl, "rne", "classes")
 
 
-def test_exponent_range_edges():
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
+def test_exponent_range_edges(p):
     rng = np.random.default_rng(23)
-    cnt = 600
+    cnt = count(p, 600)
+    N = p // 32
     lo, hi, B = 5, 0x7FFFFFFF, O.BIAS
     d = rng.integers(-3, 4, cnt)
     half = np.arange(cnt) < cnt // 2
@@ -121,13 +131,15 @@ def test_exponent_range_edges():
     al.reshape(cnt, N)[:, -1] |= np.uint32(0x80000000)
     bl.reshape(cnt, N)[:, -1] |= np.uint32(0x80000000)
     for mode in ("rne", "ru", "rd"):
-        check_oracle(ae, al, be, bl, mode, "range")
+        check_oracle(p, ae, al, be, bl, mode, "range")
 
 
-def test_matches_radix27_kernel_on_raw_limbs():
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
+def test_matches_radix27_kernel_on_raw_limbs(p):
     """Random limbs with the top bit left as drawn, plus all-zero and all-ones mantissas."""
-    rng = np.random.default_rng(29)
-    cnt = 1024
+    rng = np.random.default_rng(29 + p)
+    cnt = max(64, count(p, 1024))
+    N = p // 32
     al = rng.integers(0, 2**32, size=(cnt, N), dtype=np.uint64).astype(np.uint32)
     bl = rng.integers(0, 2**32, size=(cnt, N), dtype=np.uint64).astype(np.uint32)
     al[:8] = 0
@@ -139,9 +151,11 @@ def test_matches_radix27_kernel_on_raw_limbs():
     ae = (2 * e + rng.integers(0, 2, cnt)).astype(np.uint32)
     be = (2 * (O.BIAS + rng.integers(-100, 100, size=cnt)) + rng.integers(0, 2, cnt)).astype(np.uint32)
     for mode in ("rne", "rz", "ru", "rd", "rna"):
-        check_valu(ae, al.reshape(-1), be, bl.reshape(-1), mode, "raw limbs")
+        check_valu(p, ae, al.reshape(-1), be, bl.reshape(-1), mode, "raw limbs")
 
 
-def test_matches_radix27_kernel_large_launch():
-    ae, al, be, bl = F.generate(2, 313, "mul", P, 1 << 16)
-    check_valu(ae, al, be, bl, "rne", "2^16 values")
+@pytest.mark.parametrize("p", MFMA_PRECISIONS)
+def test_matches_radix27_kernel_large_launch(p):
+    cnt = (1 << 28) // p               # 32 MiB of limbs per operand
+    ae, al, be, bl = F.generate(2, 313, "mul", p, cnt)
+    check_valu(p, ae, al, be, bl, "rne", f"{cnt} values")
