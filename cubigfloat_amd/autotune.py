@@ -5,11 +5,14 @@ value, K*L = precision/32).  For every (op, precision) this times every shape on
 same synthetic operands (fpa_workload_generate kind 2: e ~ U[-1024, 1024], 1% zeros, 5%
 near-cancellation pairs), checks that every shape returns bit-identical results, and
 writes the fastest shape as one "op precision K L" line (op 6 = the dot-chain kernel,
-timed on chains of U[8, 512] elements).  It then times every shape again on a small
-launch (16,384 values by default, about the size of one segment of a 2^18-op mixed batch):
-such a launch cannot fill the chip and the shape with the shortest per-value latency
-wins.  Where that shape differs, a fifth field marks an extra "op precision K L max_count"
-line, used for launches of at most max_count values.  libfpa.so applies the file
+timed on chains of U[8, 512] elements).  It then times every shape again on small
+launches (2,048 to 16,384 values by default, the sizes of the segments of a 2^18-op mixed
+batch or of a served request): such a launch cannot fill the chip and the shape with the
+shortest per-value latency wins, a different one at different sizes.  Where a size's
+winner differs from the large-launch shape, "op precision K L max_count" lines (a fifth
+field) give a shape per size tier: a launch of at most max_count values takes the
+smallest tier that holds it.  A tier timed at c values reaches to the geometric middle
+between c and the next size timed (c*sqrt(2) for the largest).  libfpa.so applies the file
 when it is loaded (tuning.txt beside the library, or $FPA_TUNING_FILE), so the fpa.h
 calls, the batch graphs, the server and the dot chains all use the tuned shapes.
 
@@ -148,20 +151,36 @@ def tune_one(op, precision, count=None, seed=1, log=None):
                 x.close()
 
 
-def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None, small_count=16384):
-    """Returns ({(op, precision): ((K, L), timings, count)}, {same for small launches where
-    the small-launch winner differs}); also applies each choice."""
+SMALL_COUNTS = (2048, 4096, 8192, 16384)
+
+
+def tier_bounds(counts):
+    """Largest launch each timed size serves: the geometric middle to the next size."""
+    c = sorted(counts)
+    return [int((c[i] * c[i + 1]) ** 0.5) for i in range(len(c) - 1)] + [int(c[-1] * 2 ** 0.5)]
+
+
+def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None, small_counts=SMALL_COUNTS):
+    """Returns ({(op, precision): ((K, L), timings, count)}, {(op, precision): [((K, L),
+    timings, count, bound) per small-launch tier written]}); also applies each choice.  Tiers are
+    written up to the largest size whose winner differs from the large-launch shape (a
+    launch takes the smallest tier that holds it, so the tiers below that one are needed
+    even where their winner is the large shape)."""
     out, small = {}, {}
     for op in ops:
         for p in precisions:
             best, timings, count = tune_one(op, p, log=log)
             out[(op, p)] = (best, timings, count)
             set_variant(op, p, variants(op, p).index(best))
-            if small_count and op != "dot":
-                sbest, stimings, scount = tune_one(op, p, count=small_count, log=log)
-                if sbest != best:
-                    small[(op, p)] = (sbest, stimings, scount)
-                    set_small_variant(op, p, variants(op, p).index(sbest), small_count)
+            if not small_counts or op == "dot":
+                continue
+            bounds = tier_bounds(small_counts)
+            tiers = [tune_one(op, p, count=c, log=log) + (b,) for c, b in zip(sorted(small_counts), bounds)]
+            differ = [i for i, t in enumerate(tiers) if t[0] != best]
+            if differ:
+                small[(op, p)] = tiers[:differ[-1] + 1]
+                for sbest, _, _, bound in small[(op, p)]:
+                    set_small_variant(op, p, variants(op, p).index(sbest), bound)
     return out, small
 
 
@@ -173,11 +192,10 @@ def write_table(path, table, small=None):
         parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == best else "") for k, l, ms in timings)
         lines.append(f"# {op} {p}, {count} {'elements' if op == 'dot' else 'values'}: {parts}")
         lines.append(f"{KERNELS[op]} {p} {best[0]} {best[1]}")
-        if (op, p) in small:
-            sbest, stimings, scount = small[(op, p)]
+        for sbest, stimings, scount, bound in small.get((op, p), []):
             parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == sbest else "") for k, l, ms in stimings)
-            lines.append(f"# {op} {p}, small launches of {scount} values: {parts}")
-            lines.append(f"{KERNELS[op]} {p} {sbest[0]} {sbest[1]} {scount}")
+            lines.append(f"# {op} {p}, small launches of {scount} values (tier up to {bound}): {parts}")
+            lines.append(f"{KERNELS[op]} {p} {sbest[0]} {sbest[1]} {bound}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -187,8 +205,8 @@ def main():
     ap.add_argument("--out", required=True, help="tuning table to write")
     ap.add_argument("--ops", default=",".join(TUNED_OPS))
     ap.add_argument("--precisions", default=",".join(str(p) for p in PRECISIONS))
-    ap.add_argument("--small-count", type=int, default=16384,
-                    help="values per small launch timed for the small-launch table (0: skip)")
+    ap.add_argument("--small-counts", default=",".join(str(c) for c in SMALL_COUNTS),
+                    help="launch sizes timed for the small-launch tiers ('' to skip)")
     args = ap.parse_args()
     ops = [o for o in args.ops.split(",") if o]
     precs = [int(p) for p in args.precisions.split(",") if p]
@@ -196,9 +214,9 @@ def main():
         if op not in KERNELS:
             sys.exit(f"unknown op {op}")
     log = lambda s: print(s, flush=True)
-    table, small = tune(ops, precs, log=log, small_count=args.small_count)
+    table, small = tune(ops, precs, log=log, small_counts=[int(c) for c in args.small_counts.split(",") if c])
     write_table(args.out, table, small)
-    log(f"wrote {len(table)} entries and {len(small)} small-launch entries to {args.out}")
+    log(f"wrote {len(table)} entries and {sum(len(v) for v in small.values())} small-launch tiers to {args.out}")
 
 
 if __name__ == "__main__":

@@ -136,7 +136,12 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
   FPA_HIP(hipHostMalloc((void **)&b->mirror, b->pool_bytes, hipHostMallocDefault));
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
-    g->variant = fpa_kernel_get_variant_for_count(g->op, g->precision, g->count);
+    /* A segment shares the chip with the branches running beside it, so it takes the shape
+     * tuned for a launch twice its size: on the 2^18-op mixed batch this beat the shape for
+     * its own size (the isolated-latency winner, which takes chip share from the other
+     * branches), four times its size and the large-launch shapes (0.445 ms against 0.50,
+     * 0.49 and 0.57 ms). */
+    g->variant = fpa_kernel_get_variant_for_count(g->op, g->precision, 2 * (uint64_t)g->count);
     fpa_kernel_variant_shape(g->op, g->precision, g->variant, &g->K, &g->L);
     fpa_arguments *a = &g->args;
     a->count = g->count;

@@ -62,10 +62,14 @@ static const int prec_n[] = { 8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 512 };
 
 /* dispatch table: variant index per (op, precision); 0 = first compiled shape */
 static int variant_tab[FPA_OP_COUNT + 1][NPREC];   /* + FPA_KERNEL_DOT */
-/* optional shape for launches of at most small_max[op][p] values (0 = none): small
- * launches cannot fill the chip, so the shape with the shortest per-value latency wins */
-static int variant_small[FPA_OP_COUNT + 1][NPREC];
-static uint32_t small_max[FPA_OP_COUNT + 1][NPREC];
+/* optional shapes for small launches: tier t serves launches of at most tier_max values
+ * (tiers sorted by bound; a launch takes the first tier it fits, else variant_tab).  Small
+ * launches cannot fill 256 CUs, and the shape with the shortest per-value latency at that
+ * size wins. */
+#define FPA_SMALL_TIERS 4
+static int tier_var[FPA_OP_COUNT + 1][NPREC][FPA_SMALL_TIERS];
+static uint32_t tier_max[FPA_OP_COUNT + 1][NPREC][FPA_SMALL_TIERS];
+static int ntier[FPA_OP_COUNT + 1][NPREC];
 static void *cur_stream = NULL;
 
 static int prec_index(uint32_t precision) {
@@ -124,26 +128,43 @@ static int set_large_variant(int op, uint32_t precision, int variant) {
   return 0;
 }
 
-/* an explicit choice applies to every launch size: it also drops a small-launch variant */
+/* an explicit choice applies to every launch size: it also drops the small-launch tiers */
 int fpa_kernel_set_variant(int op, uint32_t precision, int variant) {
   if (set_large_variant(op, precision, variant) != 0) return -1;
-  small_max[op][prec_index(precision)] = 0;
+  ntier[op][prec_index(precision)] = 0;
   return 0;
 }
 
 int fpa_kernel_set_small_variant(int op, uint32_t precision, int variant, uint32_t max_count) {
   int p = prec_index(precision);
   if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
+  if (max_count == 0) {                  /* removes every tier */
+    ntier[op][p] = 0;
+    return 0;
+  }
   if (variant < 0 || variant >= fpa_kernel_variants(op, precision)) return -1;
-  variant_small[op][p] = variant;
-  small_max[op][p] = max_count;
+  int n = ntier[op][p], i;
+  for (i = 0; i < n && tier_max[op][p][i] < max_count; i++) {}
+  if (i < n && tier_max[op][p][i] == max_count) {   /* same bound: replace */
+    tier_var[op][p][i] = variant;
+    return 0;
+  }
+  if (n == FPA_SMALL_TIERS) return -1;
+  for (int j = n; j > i; j--) {          /* keep the tiers sorted by bound */
+    tier_max[op][p][j] = tier_max[op][p][j - 1];
+    tier_var[op][p][j] = tier_var[op][p][j - 1];
+  }
+  tier_max[op][p][i] = max_count;
+  tier_var[op][p][i] = variant;
+  ntier[op][p] = n + 1;
   return 0;
 }
 
 int fpa_kernel_get_variant_for_count(int op, uint32_t precision, uint64_t count) {
   int p = prec_index(precision);
   if (op < 0 || op > FPA_KERNEL_DOT || p < 0) return -1;
-  if (small_max[op][p] && count <= small_max[op][p]) return variant_small[op][p];
+  for (int i = 0; i < ntier[op][p]; i++)
+    if (count <= tier_max[op][p][i]) return tier_var[op][p][i];
   return variant_tab[op][p];
 }
 
@@ -152,12 +173,21 @@ int fpa_kernel_load_tuning(const char *path) {
   if (!f) return -1;
   char line[256];
   int applied = 0;
+  /* a file gives the whole tier set of each (op, precision) it names: the first line of a
+   * pair drops the tiers it had */
+  unsigned char seen[FPA_OP_COUNT + 1][NPREC];
+  memset(seen, 0, sizeof seen);
   while (fgets(line, sizeof line, f)) {
     int op, K, L;
     unsigned prec, maxc = 0;
     if (line[0] == '#') continue;
     int got = sscanf(line, "%d %u %d %d %u", &op, &prec, &K, &L, &maxc);
     if (got != 4 && got != 5) continue;
+    int pi = prec_index(prec);
+    if (op >= 0 && op <= FPA_KERNEL_DOT && pi >= 0 && !seen[op][pi]) {
+      seen[op][pi] = 1;
+      ntier[op][pi] = 0;
+    }
     int nv = fpa_kernel_variants(op, prec);
     for (int v = 0; v < nv; v++) {
       int k, l;

@@ -131,13 +131,25 @@ def test_small_launch_variants():
         assert F.get_variant_for_count("div", 4096, 5001) == 0
         assert F.get_variant("div", 4096) == 0
         assert F.set_small_variant("div", 4096, nv, 5000) == -1      # no such variant
+        # tiers: a launch takes the smallest bound that holds it; a repeated bound replaces
+        assert F.set_small_variant("div", 4096, 1, 2000) == 0
+        assert F.set_small_variant("div", 4096, 2 % nv, 9000) == 0
+        assert [F.get_variant_for_count("div", 4096, c) for c in (1, 2000, 2001, 5000, 5001, 9000, 9001)] == \
+            [1, 1, nv - 1, nv - 1, 2 % nv, 2 % nv, 0]
+        assert F.set_small_variant("div", 4096, 0, 2000) == 0
+        assert F.get_variant_for_count("div", 4096, 1500) == 0
+        assert F.set_small_variant("div", 4096, 0, 100) == 0        # fourth tier
+        assert F.set_small_variant("div", 4096, 0, 200) == -1       # a fifth does not fit
+        assert F.set_small_variant("div", 4096, 0, 0) == 0          # bound 0 removes every tier
+        assert F.get_variant_for_count("div", 4096, 100) == 0 and F.get_variant_for_count("div", 4096, 5000) == 0
         # an explicit override applies to every launch size
+        assert F.set_small_variant("div", 4096, nv - 1, 5000) == 0
         assert F.set_variant("div", 4096, 1) == 0
         assert F.get_variant_for_count("div", 4096, 10) == 1
     finally:
+        # restore the committed table (its tiers included)
         F.set_variant("div", 4096, keep[0])
-        if keep[1] != keep[0]:
-            F.set_small_variant("div", 4096, keep[1], 16384)
+        F.lib.fpa_kernel_load_tuning(os.path.join(ROOT, "cubigfloat_amd", "tuning.txt").encode())
 
 
 def test_scalar_helpers_vs_python():
@@ -198,14 +210,19 @@ def test_autotune_table_round_trip(tmp_path):
         pick = vs[-1]
         table[(op, p)] = (pick, [(k, l, 1.0 + i) for i, (k, l) in enumerate(vs)], 1234)
     f = tmp_path / "t.txt"
-    small = {("div", 4096): (F.variants("div", 4096)[0], [(k, l, 1.0) for k, l in F.variants("div", 4096)], 777)}
+    dv = F.variants("div", 4096)
+    small = {("div", 4096): [(dv[0], [(k, l, 1.0) for k, l in dv], 700, 777),
+                             (dv[1], [(k, l, 1.0) for k, l in dv], 2800, 3000)]}
+    assert T.tier_bounds([2048, 4096, 8192, 16384]) == [2896, 5792, 11585, 23170]
     T.write_table(str(f), table, small)
     try:
-        assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == len(table) + 1
+        assert F.lib.fpa_kernel_load_tuning(str(f).encode()) == len(table) + 2
         for (op, p) in keep:
             assert F.get_variant(op, p) == len(F.variants(op, p)) - 1
         assert F.get_variant_for_count("div", 4096, 777) == 0
-        assert F.get_variant_for_count("div", 4096, 778) == len(F.variants("div", 4096)) - 1
+        assert F.get_variant_for_count("div", 4096, 778) == 1
+        assert F.get_variant_for_count("div", 4096, 3000) == 1
+        assert F.get_variant_for_count("div", 4096, 3001) == len(dv) - 1
     finally:
         for (op, p), v in keep.items():
             F.set_variant(op, p, v)
