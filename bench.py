@@ -30,8 +30,6 @@ CUS, CLOCK_HZ, HBM_GBS = 256, 2.4e9, 8000.0
 # this; the probe below measures the device's sustained rate.)
 MAD_PEAK_T = CUS * 64 * CLOCK_HZ / 1e12
 MAD_QUARTER_RATE_T = MAD_PEAK_T / 4
-# Dense int8 matrix-core peak of MI355X (v_mfma_i32_32x32x32_i8), in tera-ops/s (2 per MAC).
-MFMA_I8_PEAK_TOPS = 5000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096.md")
 
 
@@ -392,34 +390,15 @@ def main():
     ops = head["ops_per_s"]
     traffic = measured_traffic().get("mul4096")
     shape = F.variants("mul", p)[F.get_variant("mul", p)]
-    n = p // 32
-    valu_view = {"achieved": head["tproducts_per_s"], "peak": MAD_PEAK_T, "unit": "T 32x32->64 products/s",
-                 "frac": head["tproducts_per_s"] / MAD_PEAK_T, "peak_measured_mad_probe": mad_measured_t,
-                 "frac_of_measured": head["tproducts_per_s"] / mad_measured_t,
-                 "peak_if_quarter_rate": MAD_QUARTER_RATE_T}
-    if shape[1] == 64:
-        # matrix-core kernel: the product is a (4n)-byte x (4n)-byte int8 convolution
-        nch = n // 8
-        tiles = [sum(1 for t in range((nch + 15) // 16) if 32 * t + 31 - d >= 0 and 32 * t - d <= nch - 1)
-                 for d in range(nch + 1)]
-        mfmas = sum(tiles) + 2 * sum(1 for t in range((nch + 15) // 16) if 32 * t + 31 >= nch)
-        alg_tops = 2 * (4 * n) ** 2 * ops / 1e12
-        roofline = {"bound": "mfma", "achieved": alg_tops, "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s (int8)",
-                    "frac": alg_tops / MFMA_I8_PEAK_TOPS,
-                    "issued_mfma_per_mul": mfmas,
-                    "issued_tops": mfmas * 2 * 32 * 32 * 32 * ops / 1e12,
-                    "issued_frac": mfmas * 2 * 32 * 32 * 32 * ops / 1e12 / MFMA_I8_PEAK_TOPS,
-                    "valu_view": valu_view,
-                    "note": "v_mfma_i32_32x32x32_i8 convolution of signed-digit bytes; achieved = "
-                            "algorithmic int8 work (4n)^2 = 262144 byte products = 524288 ops per mul x "
-                            "mul/s; peak = MI355X dense int8 5.0 POP/s; issued counts the MFMAs run, "
-                            "whose tiles include zero chunks; traffic = HBM bytes per launch from "
-                            "rocprofv3 FETCH_SIZE+WRITE_SIZE (profiles/)"}
-    else:
-        roofline = dict({"bound": "valu"}, **valu_view)
-        roofline["note"] = ("INT32 VALU multiply roofline; achieved = n^2 = 16384 word products per mul x "
-                            "mul/s; peak = one v_mad_u64_u32 per lane per clock, 256 CU x 64 lanes x 2.4 GHz; "
-                            "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE+WRITE_SIZE (profiles/)")
+    roofline = {"bound": "valu", "achieved": head["tproducts_per_s"], "peak": MAD_PEAK_T,
+                "unit": "T 32x32->64 products/s", "frac": head["tproducts_per_s"] / MAD_PEAK_T,
+                "peak_measured_mad_probe": mad_measured_t,
+                "frac_of_measured": head["tproducts_per_s"] / mad_measured_t,
+                "peak_if_quarter_rate": MAD_QUARTER_RATE_T,
+                "note": "INT32 VALU multiply roofline (the path is exact integer VALU work; at 4096 bits "
+                        "mul is far from the HBM bound); achieved = n^2 = 16384 word products per mul x "
+                        "mul/s; peak = one v_mad_u64_u32 per lane per clock, 256 CU x 64 lanes x 2.4 GHz; "
+                        "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE+WRITE_SIZE (profiles/)"}
     roofline["traffic"] = traffic
     roofline["traffic_algorithmic"] = algorithmic("mul", p)[1] * count
     res = {

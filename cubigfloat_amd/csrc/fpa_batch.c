@@ -64,11 +64,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static double wave_cost(int op, int K, int L) {
   int n = K * L, KD = (32 * K + 26) / 27;
   switch (op) {
-    /* the matrix-core mul (one value per wave, n/8 chunks of 32 bytes): fitted to its
-     * measured time per value relative to the radix-2^27 shapes, 2048..16384 bits */
-    case FPA_OP_MUL:
-      if (L == 64) return 0.45 * (n / 8.0) * (n / 8.0) + 14.0 * (n / 8.0);
-      return (double)L * KD * KD + 40.0 * KD;
+    case FPA_OP_MUL: return (double)L * KD * KD + 40.0 * KD;
     case FPA_OP_DIV: return (double)(n + 1) * (3 * K + 40);
     case FPA_OP_SQRT: return (double)n * (3 * K + 50);
     default: return 8.0 * K + 60.0;

@@ -40,7 +40,6 @@ int fpa_impl_launch_div(const fpa_arguments *a, int K, int L, void *stream);
 int fpa_impl_launch_sqrt(const fpa_arguments *a, int K, int L, void *stream);
 int fpa_impl_launch_sgn(uint32_t count, const uint32_t *x_exps, int32_t *r, void *stream);
 int fpa_impl_launch_setcmp(int op, const fpa_arguments *args, int K, int L, void *stream);
-int fpa_impl_launch_mul_mfma(const fpa_arguments *a, void *stream);
 
 typedef int (*launch_fn)(const fpa_arguments *, int, int, void *);
 static const launch_fn launchers[FPA_OP_COUNT] = {
@@ -48,12 +47,10 @@ static const launch_fn launchers[FPA_OP_COUNT] = {
   fpa_impl_launch_mul, fpa_impl_launch_div, fpa_impl_launch_sqrt,
 };
 
-/* compiled shape table: the element-wise kernels' shapes, then the matrix-core mul
- * kernels (listed last, so they add variants after the existing ones of a precision) */
-typedef struct { int n, K, L, mfma; } shape_t;
-#define FPA_SHAPE_ENTRY(n_, K_, L_) { n_, K_, L_, 0 },
-#define FPA_MFMA_SHAPE_ENTRY(n_, K_, L_) { n_, K_, L_, 1 },
-static const shape_t shapes[] = { FPA_SHAPES(FPA_SHAPE_ENTRY) FPA_MFMA_SHAPES(FPA_MFMA_SHAPE_ENTRY) };
+/* compiled shape table */
+typedef struct { int n, K, L; } shape_t;
+#define FPA_SHAPE_ENTRY(n_, K_, L_) { n_, K_, L_ },
+static const shape_t shapes[] = { FPA_SHAPES(FPA_SHAPE_ENTRY) };
 #define NSHAPES ((int)(sizeof(shapes) / sizeof(shapes[0])))
 
 /* supported limb counts, in table order */
@@ -81,12 +78,9 @@ static int prec_index(uint32_t precision) {
 
 int fpa_precision_supported(uint32_t precision) { return prec_index(precision) >= 0; }
 
-/* shapes an op may use: every element-wise one, except that the dot kernel exists only
- * for K <= FPA_DOT_MAX_K; the matrix-core kernels serve mul only */
-static int shape_allowed(int op, int s) {
-  if (shapes[s].mfma) return op == FPA_OP_MUL;
-  return op != FPA_KERNEL_DOT || shapes[s].K <= FPA_DOT_MAX_K;
-}
+/* shapes an op may use: every compiled one, except that the dot kernel exists only for
+ * K <= FPA_DOT_MAX_K */
+static int shape_allowed(int op, int s) { return op != FPA_KERNEL_DOT || shapes[s].K <= FPA_DOT_MAX_K; }
 
 /* variant v of (op, limb count n) -> index into shapes[], or -1 */
 static int shape_of(int op, int n, int v) {
@@ -208,7 +202,6 @@ int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stre
   int s = prec_index(args->precision) < 0 ? -1 : shape_of(op, (int)(args->precision / 32), variant);
   if (s < 0) return -1;
   if (args->count == 0) return 0;
-  if (shapes[s].mfma) return fpa_impl_launch_mul_mfma(args, stream);
   return launchers[op](args, shapes[s].K, shapes[s].L, stream);
 }
 

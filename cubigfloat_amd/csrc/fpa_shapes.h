@@ -20,10 +20,6 @@
   X(256, 16, 16) X(256, 32, 8)  X(256, 8, 32)                \
   X(512, 16, 32) X(512, 32, 16)
 
-/* Matrix-core mul kernels (op mul only): (n, K, L) with one value per wave, L = 64 lanes
- * holding K limbs each on input; the product is computed with int8 MFMAs. */
-#define FPA_MFMA_SHAPES(X) X(64, 1, 64) X(128, 2, 64) X(192, 3, 64) X(256, 4, 64) X(512, 8, 64)
-
 /* The dot-chain kernel (one mul core and one add core per element) is compiled only for
  * shapes with K <= FPA_DOT_MAX_K: above that its add core no longer inlines and becomes a
  * real call, which no kernel of this library may contain. */

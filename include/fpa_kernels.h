@@ -68,14 +68,16 @@ int fpa_kernel_set_variant(int op, uint32_t precision, int variant);
 
 /* The variant tables are process-wide: set them before launching from several threads. */
 
-/* Small launches: a second variant used for launches of at most max_count values
- * (max_count 0 removes it), and the variant the dispatcher uses for a launch of `count`
- * values (the fpa.h calls, batch segments, server groups and dot chains all ask this). */
+/* Small launches: up to four tiers of (max_count, variant) per (op, precision); a launch of
+ * `count` values takes the tier with the smallest max_count >= count, else the variant
+ * above.  Setting a tier with an existing max_count replaces it; max_count 0 removes every
+ * tier; -1 if the variant is invalid or four other tiers exist.  get_variant_for_count is
+ * what the fpa.h calls, batch segments, server groups and dot chains all ask. */
 int fpa_kernel_set_small_variant(int op, uint32_t precision, int variant, uint32_t max_count);
 int fpa_kernel_get_variant_for_count(int op, uint32_t precision, uint64_t count);
 
 /* Load "op precision K L" lines (as written by the autotuner) into the dispatch table;
- * a fifth field max_count makes the line a small-launch variant.
+ * a fifth field max_count makes the line a small-launch tier.
  * Returns the number of entries applied, or -1 if the file cannot be read.
  * When the library is loaded it applies, if present, the file named by the environment
  * variable FPA_TUNING_FILE, else tuning.txt in the directory holding the library
