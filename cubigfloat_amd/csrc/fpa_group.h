@@ -120,6 +120,72 @@ __device__ __forceinline__ uint32_t chunk_sub_word(uint32_t (&x)[K], uint32_t w)
   return c;
 }
 
+/* x += w at word 0, the carry taken through limbs 0..H-1 only; returns the carry out of
+ * limb H-1.  After adding a full word to limb 0, a carry leaves limb 1 only if limb 1 was
+ * all ones, so a two-limb head almost always settles the sum; chunk_carry_tail finishes. */
+template <int K, int H>
+__device__ __forceinline__ uint32_t chunk_add_word_head(uint32_t (&x)[K], uint32_t w) {
+  static_assert(H >= 1 && H <= K, "head within the chunk");
+  unsigned int c, o;
+  x[0] = __builtin_addc(x[0], w, 0u, &o);
+  c = o;
+  if constexpr (H > 1) {
+    const uint32_t z = opaque_zero();
+#pragma unroll
+    for (int i = 1; i < H; i++) {
+      x[i] = __builtin_addc(x[i], z, c, &o);
+      c = o;
+    }
+  }
+  return c;
+}
+
+/* x -= w at word 0, the borrow taken through limbs 0..H-1 only; returns the borrow out of
+ * limb H-1 (see chunk_add_word_head) */
+template <int K, int H>
+__device__ __forceinline__ uint32_t chunk_sub_word_head(uint32_t (&x)[K], uint32_t w) {
+  static_assert(H >= 1 && H <= K, "head within the chunk");
+  unsigned int c, o;
+  x[0] = __builtin_subc(x[0], w, 0u, &o);
+  c = o;
+  if constexpr (H > 1) {
+    const uint32_t z = opaque_zero();
+#pragma unroll
+    for (int i = 1; i < H; i++) {
+      x[i] = __builtin_subc(x[i], z, c, &o);
+      c = o;
+    }
+  }
+  return c;
+}
+
+/* carry (or, with Sub, borrow) c into limb H, through limbs H..K-1; returns the one out */
+template <int K, int H, bool Sub>
+__device__ __forceinline__ uint32_t chunk_carry_tail(uint32_t (&x)[K], uint32_t c) {
+  if constexpr (H < K) {
+    const uint32_t z = opaque_zero();
+    unsigned int o;
+#pragma unroll
+    for (int i = H; i < K; i++) {
+      x[i] = Sub ? __builtin_subc(x[i], z, c, &o) : __builtin_addc(x[i], z, c, &o);
+      c = o;
+    }
+  }
+  return c;
+}
+
+/* x += w / x -= w at word 0 in every lane of the wave: the head, then the tail only where
+ * some lane of the wave carries out of the head; returns this lane's carry/borrow out */
+template <int K, bool Sub>
+__device__ __forceinline__ uint32_t chunk_word_wave(uint32_t (&x)[K], uint32_t w) {
+  constexpr int H = K < 2 ? K : 2;
+  uint32_t c = Sub ? chunk_sub_word_head<K, H>(x, w) : chunk_add_word_head<K, H>(x, w);
+  if constexpr (H < K) {
+    if (__builtin_expect(__ballot(c != 0) != 0, 0)) c = chunk_carry_tail<K, H, Sub>(x, c);
+  }
+  return c;
+}
+
 template <int K>
 __device__ __forceinline__ void chunk_select(uint32_t (&r)[K], bool c, const uint32_t (&a)[K], const uint32_t (&b)[K]) {
 #pragma unroll
@@ -262,7 +328,7 @@ struct Group {
     uint32_t c = chunk_add(r, a, b, 0u);
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(r) == 0xFFFFFFFFu, cin, cout);
-    chunk_add_word(r, ci);
+    chunk_word_wave<K, false>(r, ci);
     return cout;
   }
 
@@ -273,7 +339,7 @@ struct Group {
     uint32_t c = chunk_sub(r, a, b, 0u);
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(r) == 0u, bin, bout);
-    chunk_sub_word(r, bi);
+    chunk_word_wave<K, true>(r, bi);
     return bout;
   }
 
@@ -281,27 +347,27 @@ struct Group {
    * returns the carry out of the value */
   template <int K>
   __device__ __forceinline__ uint32_t add_lane_words(uint32_t (&x)[K], uint32_t w) const {
-    uint32_t c = chunk_add_word(x, w);
+    uint32_t c = chunk_word_wave<K, false>(x, w);
     /* A chunk passes a carry on only if adding w overflowed all its limbs: rare.  When no
      * lane of the wave did, no carry enters any lane and none leaves any value (the
      * lookahead below would give all zeros), so the resolution is skipped. */
     if (__builtin_expect(__ballot(c != 0) == 0, 1)) return 0u;
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(x) == 0xFFFFFFFFu, 0u, cout);
-    if (__ballot(ci != 0) != 0) chunk_add_word(x, ci);
+    if (__ballot(ci != 0) != 0) chunk_word_wave<K, false>(x, ci);
     return cout;
   }
 
   /* x -= w_lane * 2^(32*K*gl); returns the borrow out of the value */
   template <int K>
   __device__ __forceinline__ uint32_t sub_lane_words(uint32_t (&x)[K], uint32_t w) const {
-    uint32_t c = chunk_sub_word(x, w);
+    uint32_t c = chunk_word_wave<K, true>(x, w);
     /* a chunk passes a borrow on only if its limbs above word 0 were all zero: rare; with
      * no such lane in the wave nothing propagates (see add_lane_words) */
     if (__builtin_expect(__ballot(c != 0) == 0, 1)) return 0u;
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(x) == 0u, 0u, bout);
-    if (__ballot(bi != 0) != 0) chunk_sub_word(x, bi);
+    if (__ballot(bi != 0) != 0) chunk_word_wave<K, true>(x, bi);
     return bout;
   }
 
