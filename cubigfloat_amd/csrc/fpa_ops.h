@@ -331,6 +331,33 @@ __device__ __forceinline__ uint32_t dpp_row_next(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x101, 0xF, 0xF, true);
 }
 
+/* own + (nb of lane+1 within this lane's 16-lane row; 0 in the row's lane 15), mod 2^64.
+ * The compiler moves each DPP operand with a v_mov_b32_dpp of its own and does not fold it
+ * into a carry add, so both halves are written here as VOP2 adds whose first source reads
+ * the neighbour through DPP: two VALU instead of four.  The compiler counts no wait states
+ * for asm, so the block keeps its own: 2 before its DPP reads (a VALU write of a source
+ * just before), 5 when Nop says so (the first block after an EXEC write in the same
+ * basic block), and 2 after it when Tail says so (a DPP read of its results just after). */
+#define FPA_ADD_ROW_NEXT64(pre, post)                                                            \
+  asm(pre "\n\t"                                                                                  \
+      "v_add_co_u32_dpp %0, vcc, %2, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"    \
+      "v_addc_co_u32_dpp %1, vcc, %4, %5, vcc row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1" \
+      post                                                                                        \
+      : "=&v"(lo), "=v"(hi)                                                                       \
+      : "v"((uint32_t)nb), "v"((uint32_t)own), "v"((uint32_t)(nb >> 32)), "v"((uint32_t)(own >> 32)) \
+      : "vcc")
+template <int Nop, bool Tail>
+__device__ __forceinline__ uint64_t add_row_next64(uint64_t own, uint64_t nb) {
+  static_assert(Nop == 1 || Nop == 4, "s_nop count");
+  uint32_t lo, hi;
+  if constexpr (Nop == 4 && Tail) FPA_ADD_ROW_NEXT64("s_nop 4", "\n\ts_nop 1");
+  else if constexpr (Nop == 4) FPA_ADD_ROW_NEXT64("s_nop 4", "");
+  else if constexpr (Tail) FPA_ADD_ROW_NEXT64("s_nop 1", "\n\ts_nop 1");
+  else FPA_ADD_ROW_NEXT64("s_nop 1", "");
+  return ((uint64_t)hi << 32) | lo;
+}
+#undef FPA_ADD_ROW_NEXT64
+
 /* 32 bits at bit offset o >= 0 of a digit string (radix 2^27) held in LDS; the two words
  * past its top digit must read as zero */
 __device__ __forceinline__ uint32_t digits_bits(const uint32_t *d, int o) {
@@ -462,21 +489,35 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
         /* Lockstep: a group's top lane reads lane 0 of the next group in its row (not
          * zero); that value is the next group's complete chunk r, and it ends up,
          * unchanged, in lane r's final window, where it is subtracted below. */
-        const bool top = !kLockstep && g.gl == L - 1;
+        if constexpr (kLockstep) {
+          /* the blocks all write VCC, so they stay in this order: the first follows the
+           * EXEC restore after lane 0's stores, the last precedes code that may read its
+           * results through DPP */
 #pragma unroll
-        for (int i = 0; i < KD; i++) {
-          uint32_t lo = dpp_row_next((uint32_t)W[i]);
-          uint32_t hi = dpp_row_next((uint32_t)(W[i] >> 32));
-          if (!kLockstep) {
+          for (int i = 0; i < KD; i++) {
+            if (KD == 1) W[i] = add_row_next64<4, true>(W[KD + i], W[i]);
+            else if (i == 0) W[i] = add_row_next64<4, false>(W[KD + i], W[i]);
+            else if (i == KD - 1) W[i] = add_row_next64<1, true>(W[KD + i], W[i]);
+            else W[i] = add_row_next64<1, false>(W[KD + i], W[i]);
+            W[KD + i] = 0;
+          }
+        } else {
+          /* every lane moves its neighbour's words first and the top lane zeroes them
+           * after: written as a select of the moved value, the compiler branches around
+           * the DPP moves for the top lane */
+          const bool top = g.gl == L - 1;
+#pragma unroll
+          for (int i = 0; i < KD; i++) {
+            uint32_t lo = dpp_row_next((uint32_t)W[i]);
+            uint32_t hi = dpp_row_next((uint32_t)(W[i] >> 32));
             lo = top ? 0u : lo;
             hi = top ? 0u : hi;
+            unsigned int c, c2;
+            const uint32_t nlo = __builtin_addc(lo, (uint32_t)W[KD + i], 0u, &c);
+            const uint32_t nhi = __builtin_addc(hi, (uint32_t)(W[KD + i] >> 32), c, &c2);
+            W[i] = ((uint64_t)nhi << 32) | nlo;
+            W[KD + i] = 0;
           }
-          /* one add-with-carry per half, so each row shift can fold into its add */
-          unsigned int c, c2;
-          const uint32_t nlo = __builtin_addc(lo, (uint32_t)W[KD + i], 0u, &c);
-          const uint32_t nhi = __builtin_addc(hi, (uint32_t)(W[KD + i] >> 32), c, &c2);
-          W[i] = ((uint64_t)nhi << 32) | nlo;
-          W[KD + i] = 0;
         }
       } else {
         /* the group spans DPP rows: next window = own high half + next lane's low half */
