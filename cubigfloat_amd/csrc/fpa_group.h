@@ -34,6 +34,15 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+/* f(0), f(1), ..., f(N-1) as straight-line code (no loop for the unroller to decline) */
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void repeat(F &&f) {
+  if constexpr (I < N) {
+    f(I);
+    repeat<N, I + 1>(f);
+  }
+}
+
 template <int K>
 __device__ __forceinline__ uint32_t or_all(const uint32_t (&x)[K]) {
   uint32_t r = 0;

@@ -664,6 +664,14 @@ __device__ __forceinline__ bool top_negative(const Group<L> &g, int64_t rtop) {
 }
 
 /* -------------------------------- div -------------------------------- */
+#ifndef FPA_DIV_UNROLL
+#define FPA_DIV_UNROLL 16
+#endif
+__host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+/* digit steps per loop trip in div and sqrt (a divisor of K, so the trips cover n = K*L
+ * steps) */
+__host__ __device__ constexpr int div_steps_per_trip(int K) { return gcd_int(K, FPA_DIV_UNROLL); }
+
 /* (re, Qr) = a / b; lds: n+2 words (quotient digits) */
 template <int K, int L>
 __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
@@ -691,7 +699,7 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
   const uint32_t d1 = g.template word<K, n - 1>(B), d0 = g.template word<K, n - 2>(B);
   const uint32_t v = fpa_recip_3by2(d1, d0);
 
-  for (int j = 0; j <= n; j++) {
+  auto step = [&](int j) __attribute__((always_inline)) {
     /* R <- R * 2^32; the word leaving the top becomes rtop (top lane only) */
     int64_t rtop = R[K - 1];
     uint32_t in = g.from_prev(R[K - 1]);
@@ -715,7 +723,14 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
       }
     }
     if (g.gl == 0) lds[n - j] = q;
-  }
+  };
+  /* U steps per trip: a step moves every word of R up one place, which in straight-line
+   * code is only a renaming, so R's registers rotate and the loop's back edge moves each
+   * word once per U steps instead of once per step */
+  constexpr int U = div_steps_per_trip(K);
+  int j0 = 0;
+  for (; j0 + U <= n + 1; j0 += U) repeat<U>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+  repeat<(n + 1) % U>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
   const uint32_t rem_nz = g.any(or_all(R) != 0) ? 1u : 0u;
   wsync();
   /* quotient digits Q[0..n] (Q[n] most significant), plus the leading bit */
@@ -807,7 +822,7 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     wsync();
   }
   uint32_t v = 0;
-  for (int pos = n - 2; pos >= 0; pos--) {
+  auto step = [&](int pos) __attribute__((always_inline)) {
     /* R <- R * 2^32; rtop (top lane only) takes the word leaving the top */
     uint32_t rt = R[K - 1];
     uint32_t in = g.from_prev(R[K - 1]);
@@ -860,7 +875,12 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
       Sl[pos] = d;
     }
     wsync();
-  }
+  };
+  /* digits n-2 down to 0, U per trip (see div) */
+  constexpr int U = div_steps_per_trip(K);
+  int p0 = n - 2;
+  for (; p0 + 1 >= U; p0 -= U) repeat<U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
+  repeat<(n - 1) % U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
   /* root S (n words), remainder (rtop : R) = X - S^2 */
   uint32_t S[K], D[K];
   lds_load(S, Sl + g.gl * K);
