@@ -384,12 +384,26 @@ __device__ __forceinline__ void recode_digits(const Group<L> &g, uint32_t *lds, 
     for (int i = 0; i < PADW; i++) lds[n + i] = 0;
   }
   wsync();
+  /* the lane's digits are bits [b, b + 27*KD): funnel-shift NW words starting at bit b
+   * out of LDS once, then every digit sits at a fixed offset of that window.  The highest
+   * word read, floor(b/32) + NW, is at most ceil(27*KD*L/32) = n + PADW - 1. */
+  constexpr int NW = (MUL_RADIX * KD + 31) / 32;
   const int b = MUL_RADIX * KD * g.gl;
+  const uint32_t *src = lds + (b >> 5);
+  const uint32_t s = (uint32_t)b & 31u;
+  uint32_t win[NW + 1];
+  uint32_t lo = src[0];
+#pragma unroll
+  for (int i = 0; i < NW; i++) {
+    const uint32_t hi = src[i + 1];
+    win[i] = __builtin_amdgcn_alignbit(hi, lo, s);
+    lo = hi;
+  }
+  win[NW] = 0;
 #pragma unroll
   for (int k = 0; k < KD; k++) {
-    const int pos = b + MUL_RADIX * k;
-    const int w = pos >> 5;
-    D[k] = __builtin_amdgcn_alignbit(lds[w + 1], lds[w], (uint32_t)pos & 31u) & M;
+    const int j = (MUL_RADIX * k) >> 5, sh = (MUL_RADIX * k) & 31;
+    D[k] = (sh + MUL_RADIX <= 32 ? win[j] >> sh : __builtin_amdgcn_alignbit(win[j + 1], win[j], (uint32_t)sh)) & M;
   }
   wsync();
 }
@@ -418,6 +432,34 @@ __device__ __forceinline__ uint32_t add_carry_digits(uint32_t (&dg)[KD], uint64_
     c = v >> MUL_RADIX;
   }
   return (uint32_t)c;
+}
+
+/* add_carry_digits for a carry below 2^54, called by every lane of the wave together.
+ * Digit 0 passes on at most 2^27 and digit 1 then at most 1, which moves on only past a
+ * digit of all ones, so the rest of the chunk is visited only when some lane of the wave
+ * still carries after two digits. */
+template <int KD>
+__device__ __forceinline__ uint32_t add_carry_digits_wave(uint32_t (&dg)[KD], uint64_t c) {
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+  constexpr int H = KD < 2 ? KD : 2;
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    uint64_t v = (uint64_t)dg[k] + c;
+    dg[k] = (uint32_t)v & M;
+    c = v >> MUL_RADIX;
+  }
+  uint32_t c32 = (uint32_t)c;
+  if constexpr (H < KD) {
+    if (__builtin_expect(__ballot(c32 != 0) != 0, 0)) {
+#pragma unroll
+      for (int k = H; k < KD; k++) {
+        const uint32_t v = dg[k] + c32;
+        dg[k] = v & M;
+        c32 = v >> MUL_RADIX;
+      }
+    }
+  }
+  return c32;
 }
 
 /* (re, R) = a * b; lds: mul_lds_words(K, L) words (64-bit low columns, then digits; a's
@@ -568,8 +610,9 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
     uint64_t top_l = ((uint64_t)g.bcast((uint32_t)(co_l >> 32), L - 1) << 32) | g.bcast((uint32_t)co_l, L - 1);
     uint64_t in_l = ((uint64_t)pl_hi << 32) | pl_lo;
     uint64_t in_h = g.gl == 0 ? top_l : (((uint64_t)ph_hi << 32) | ph_lo);
-    uint32_t ol = add_carry_digits<KD>(dl, in_l);
-    uint32_t oh = add_carry_digits<KD>(dh, in_h);
+    /* chunk carries are below 2^38: a column sum is below 2^64 */
+    uint32_t ol = add_carry_digits_wave<KD>(dl, in_l);
+    uint32_t oh = add_carry_digits_wave<KD>(dh, in_h);
     /* 1-bit carries along the chain: bit t = low chunk t, bit L+t = high chunk t */
     bool pl = true, ph = true;
 #pragma unroll
@@ -601,15 +644,21 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
 #pragma unroll
   for (int i = 0; i < K; i++) H[i] = digits_bits(lds, base + 32 * (g.gl * K + i));
   uint32_t rw = digits_bits(lds, base - 32);
-  uint32_t loc = 0;
+  uint32_t loc;
   {
-    const int lim = base - 32;               /* sticky covers bits [0, lim) */
+    /* sticky covers bits [0, lim), lim = base - 32 in {p - 33, p - 32}: low digits below
+     * digit D0 = (p - 33) / 27 are whole, digit D0 holds lim (p - 32 - 27*D0 <= 27), and
+     * D0 sits in lane D0 / KD at index D0 % KD */
+    constexpr int D0 = (p - 33) / MUL_RADIX, lane_b = D0 / KD, i_b = D0 % KD;
+    const int lim = base - 32;
+    uint32_t pre = 0, rest = 0;
 #pragma unroll
     for (int i = 0; i < KD; i++) {
-      int lo_bit = MUL_RADIX * (g.gl * KD + i);
-      if (lo_bit + MUL_RADIX <= lim) loc |= dl[i];
-      else if (lo_bit < lim) loc |= dl[i] & ((1u << (lim - lo_bit)) - 1u);
+      if (i < i_b) pre |= dl[i];
+      else rest |= dl[i];
     }
+    const uint32_t part = dl[i_b] & ((1u << (lim - MUL_RADIX * D0)) - 1u);
+    loc = g.gl < lane_b ? (pre | rest) : g.gl == lane_b ? (pre | part) : 0u;
   }
   const uint32_t sticky = g.any(loc != 0) ? 1u : 0u;
   wsync();
