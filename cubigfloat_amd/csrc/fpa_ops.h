@@ -337,10 +337,13 @@ __device__ __forceinline__ uint32_t dpp_row_next(uint32_t x) {
  * the neighbour through DPP: two VALU instead of four.  The compiler counts no wait states
  * for asm, so the block keeps its own: 2 before its DPP reads (a VALU write of a source
  * just before), 5 when Nop says so (the first block after an EXEC write in the same
- * basic block), and 2 after it when Tail says so (a DPP read of its results just after). */
+ * basic block), 2 between the VCC write of the low add and its read as the high add's
+ * carry (the compiler keeps the same 2 between its own carry pairs on this target), and
+ * 2 after it when Tail says so (a DPP read of its results just after). */
 #define FPA_ADD_ROW_NEXT64(pre, post)                                                            \
   asm(pre "\n\t"                                                                                  \
       "v_add_co_u32_dpp %0, vcc, %2, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"    \
+      "s_nop 1\n\t"                                                                              \
       "v_addc_co_u32_dpp %1, vcc, %4, %5, vcc row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1" \
       post                                                                                        \
       : "=&v"(lo), "=v"(hi)                                                                       \
@@ -599,8 +602,9 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
   uint32_t dl[KD], dh[KD];
   {
     uint64_t cl[KD], ch[KD];
+    const uint64_t *lrow = low + g.gl * KD;     /* one base address, constant offsets */
 #pragma unroll
-    for (int i = 0; i < KD; i++) { cl[i] = low[g.gl * KD + i]; ch[i] = W[i]; }
+    for (int i = 0; i < KD; i++) { cl[i] = lrow[i]; ch[i] = W[i]; }
     wsync();   /* every lane has read its low columns before the digits overwrite them */
     uint64_t co_l = normalize_columns<KD>(cl, dl, 0);
     uint64_t co_h = normalize_columns<KD>(ch, dh, 0);
