@@ -821,7 +821,9 @@ __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g,
 }
 
 /* -------------------------------- sqrt -------------------------------- */
-/* lds: 2n+2 words: T = 2S (+ current digit) at [0, n+1), root digits at [n+1, 2n+1) */
+/* lds: n+1 words: T = 2S (+ current digit) at [0, n+1).  The root is T/2 at the end (the
+ * bit above T's n words is dtop), so no separate root array is kept: a smaller LDS
+ * footprint lets more waves share a CU. */
 template <int K, int L>
 __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                           uint32_t &re, uint32_t (&Sr)[K]) {
@@ -834,7 +836,7 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     for (int i = 0; i < K; i++) Sr[i] = 0;
     return;
   }
-  uint32_t *Tl = lds, *Sl = lds + n + 1;
+  uint32_t *Tl = lds;
   uint32_t R[K];
   int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
   uint32_t shift_in = 0;
@@ -871,7 +873,7 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     rtop = (int64_t)(rem >> 32);
     dtop = d1 >> 31;
     wsync();
-    if (g.gl == 0) { Tl[n - 1] = d1 << 1; Sl[n - 1] = d1; }
+    if (g.gl == 0) Tl[n - 1] = d1 << 1;
     wsync();
   }
   uint32_t v = 0;
@@ -925,7 +927,6 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     if (g.gl == 0) {
       Tl[pos] = d << 1;
       Tl[pos + 1] |= d >> 31;
-      Sl[pos] = d;
     }
     wsync();
   };
@@ -934,9 +935,16 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
   int p0 = n - 2;
   for (; p0 + 1 >= U; p0 -= U) repeat<U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
   repeat<(n - 1) % U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
-  /* root S (n words), remainder (rtop : R) = X - S^2 */
+  /* root S = (dtop : T) / 2 (n words), remainder (rtop : R) = X - S^2 */
   uint32_t S[K], D[K];
-  lds_load(S, Sl + g.gl * K);
+  {
+    uint32_t T[K];
+    lds_load(T, Tl + g.gl * K);
+    const uint32_t up = g.from_next(T[0], dtop);    /* the top lane takes dtop */
+#pragma unroll
+    for (int i = 0; i < K - 1; i++) S[i] = __builtin_amdgcn_alignbit(T[i + 1], T[i], 1u);
+    S[K - 1] = __builtin_amdgcn_alignbit(up, T[K - 1], 1u);
+  }
   rtop = (int64_t)(((uint64_t)g.bcast((uint32_t)((uint64_t)rtop >> 32), L - 1) << 32) |
                    g.bcast((uint32_t)rtop, L - 1));
   const uint32_t rem_nz = (rtop != 0 || g.any(or_all(R) != 0)) ? 1u : 0u;
