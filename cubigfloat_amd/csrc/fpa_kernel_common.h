@@ -10,13 +10,18 @@
 #include "fpa_ops.h"
 #include "fpa_shapes.h"
 
-/* Optional occupancy target (waves per SIMD) for every element-wise kernel, set at build
- * time (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n); unset = the compiler's own choice. */
+/* Occupancy target (waves per SIMD) of an element-wise kernel.  sqrt with 32 limbs per
+ * lane needs 129-140 VGPRs left alone, which allows 3 waves per SIMD; held to 128 it
+ * does not spill and runs 4.  Others take the compiler's own choice (1 = no target),
+ * unless a build sets one for every kernel (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
+__host__ __device__ constexpr int fpa_waves_per_eu(int op, int K) {
 #ifdef FPA_WAVES_PER_EU
-#define FPA_KERNEL_ATTRS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FPA_WAVES_PER_EU)))
+  return (void)op, (void)K, FPA_WAVES_PER_EU;
 #else
-#define FPA_KERNEL_ATTRS __launch_bounds__(256)
+  return op == 5 && K == 32 ? 4 : 1;
 #endif
+}
+#define FPA_KERNEL_ATTRS(OP, K) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K))))
 
 namespace fpa {
 
@@ -33,7 +38,7 @@ struct ScratchWords {
 };
 
 template <int K, int L, int OP>
-__global__ FPA_KERNEL_ATTRS void fpa_kernel(fpa_arguments a) {
+__global__ FPA_KERNEL_ATTRS(OP, K) void fpa_kernel(fpa_arguments a) {
   constexpr int VPW = 64 / L;
   constexpr int SW = ScratchWords<OP, K, L>::value;
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
