@@ -821,11 +821,78 @@ __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g,
 }
 
 /* -------------------------------- sqrt -------------------------------- */
+/* shapes whose first n/2 - 1 digit steps run on the frame's top half (K/2 words per lane):
+ * the steps per loop trip must be a multiple of K/2.  K = 4 (two words per lane in the
+ * half) is left on the whole-frame loop: built that way, 4x8 gave a wrong root for one
+ * input of the reference's generator at 1024 bits (tests/test_gpu_parity.py pins it) and
+ * the cause was not found. */
+__host__ __device__ constexpr bool sqrt_windowed(int K) {
+  return K >= 8 && K % 2 == 0 && div_steps_per_trip(K) % (K / 2) == 0;
+}
+
+/* One root digit (position pos) on R, KC words per lane holding frame words
+ * [n - KC*L, n): R <- R*2^32 (in0 enters at group lane 0's bottom, the top word goes to
+ * rtop), digit estimate from (R/2)/S's top words in the top lane, broadcast,
+ * R -= d*(T + d*2^(32 pos)) with T = 2S read from LDS (Tl, frame positions), add-back
+ * while negative, then T gets the digit. */
+template <int KC, int K, int L>
+__device__ __forceinline__ void sqrt_step(const Group<L> &g, uint32_t *Tl, int pos, uint32_t (&R)[KC], uint32_t in0,
+                                          int64_t &rtop, uint32_t dtop, uint32_t &v) {
+  constexpr int n = K * L, B = n - KC * L;
+  const uint32_t rt = R[KC - 1];
+  const uint32_t in = g.from_prev(R[KC - 1], in0);
+#pragma unroll
+  for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
+  R[0] = in;
+  rtop = (rtop << 32) | rt;
+  const uint32_t r1 = R[KC - 1], r2 = R[KC - 2];
+  const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
+  const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
+  const uint32_t u0 = (r1 << 31) | (r2 >> 1);
+  const uint32_t t1 = Tl[n - 1], t2 = Tl[n - 2];
+  const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
+  if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
+  uint32_t d;
+  if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
+  else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
+  d = g.bcast(d, L - 1);
+  if (g.gl == 0) Tl[pos] = d;
+  wsync();
+  uint32_t Tr[KC];
+  lds_load(Tr, Tl + B + g.gl * KC);
+  msub_step<KC, L>(g, R, rtop, d, Tr, dtop);
+  if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
+    /* add back (the estimate exceeded the digit): rare, off the common path */
+    for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
+      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
+      uint32_t c = g.add(R, R, Tr);
+      rtop += (int64_t)c + dtop;
+      d -= 1u;
+      uint32_t Y[KC];
+#pragma unroll
+      for (int i = 0; i < KC; i++) {
+        bool here = B + g.gl * KC + i == pos;
+        Y[i] = here ? d : 0u;
+        Tr[i] = here ? d : Tr[i];
+      }
+      c = g.add(R, R, Y);
+      rtop += (int64_t)c;
+    }
+  }
+  wsync();
+  /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */
+  if (g.gl == 0) {
+    Tl[pos] = d << 1;
+    Tl[pos + 1] |= d >> 31;
+  }
+  wsync();
+}
+
 /* lds: n+1 words: T = 2S (+ current digit) at [0, n+1).  The root is T/2 at the end (the
  * bit above T's n words is dtop), so no separate root array is kept: a smaller LDS
  * footprint lets more waves share a CU. */
 template <int K, int L>
-__device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+__device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                           uint32_t &re, uint32_t (&Sr)[K]) {
   static_assert(K >= 2, "the top lane must hold the two leading words after a shift");
   constexpr int n = K * L;
@@ -877,64 +944,69 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
     wsync();
   }
   uint32_t v = 0;
-  auto step = [&](int pos) __attribute__((always_inline)) {
-    /* R <- R * 2^32; rtop (top lane only) takes the word leaving the top */
-    uint32_t rt = R[K - 1];
-    uint32_t in = g.from_prev(R[K - 1]);
+  constexpr int U = div_steps_per_trip(K);
+  if constexpr (sqrt_windowed(K)) {
+    /* Digits n-2 .. n/2 change only words [n/2, n) of the frame: the words below are
+     * radicand words moving up (then zeros).  These digits run on that top half spread
+     * over all L lanes, KH = K/2 words each (W: frame words n/2 + gl*KH + i), and each
+     * step's shift takes in the radicand word below from X (frame words gl*KH + i as they
+     * stood after the first digit).  Then the frame is rebuilt and digits n/2-1 .. 0 run
+     * on all of it. */
+    constexpr int KH = K / 2, NH = n / 2;
+    uint32_t W[KH], X[KH];
+    {
+      /* frame word q (lane q/K, index q%K) -> half-layout lane q/KH, index q%KH */
+      const int src = (L == 1) ? 0 : ((L + g.gl) >> 1);
+      const bool upper = ((L + g.gl) & 1) != 0;
+      const int srcx = g.gl >> 1;
+      const bool upperx = (g.gl & 1) != 0;
 #pragma unroll
-    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
-    R[0] = in;
-    rtop = (rtop << 32) | rt;
-    /* estimate: floor((R/2) / S) from the top words, S = T/2; the top lane holds them
-     * and its digit is broadcast */
-    const uint32_t r1 = R[K - 1], r2 = R[K - 2];
-    const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
-    const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
-    const uint32_t u0 = (r1 << 31) | (r2 >> 1);
-    const uint32_t t1 = Tl[n - 1], t2 = Tl[n - 2];
-    const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
-    if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
-    uint32_t d;
-    if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
-    else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
-    d = g.bcast(d, L - 1);
-    /* R -= d * (T + d*2^(32 pos)) */
-    if (g.gl == 0) Tl[pos] = d;
-    wsync();
-    uint32_t Tr[K];
-    lds_load(Tr, Tl + g.gl * K);
-    msub_step<K, L>(g, R, rtop, d, Tr, dtop);
-    if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
-      /* add back (the estimate exceeded the digit): rare, off the common path */
-      for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
-        /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
-        uint32_t c = g.add(R, R, Tr);
-        rtop += (int64_t)c + dtop;
-        d -= 1u;
-        uint32_t Y[K];
-#pragma unroll
-        for (int i = 0; i < K; i++) {
-          bool here = g.gl * K + i == pos;
-          Y[i] = here ? d : 0u;
-          Tr[i] = here ? d : Tr[i];
+      for (int i = 0; i < KH; i++) {
+        if constexpr (L == 1) {
+          W[i] = R[KH + i];
+          X[i] = R[i];
+        } else {
+          const uint32_t wl = g.bcast(R[i], src), wh = g.bcast(R[KH + i], src);
+          const uint32_t xl = g.bcast(R[i], srcx), xh = g.bcast(R[KH + i], srcx);
+          W[i] = upper ? wh : wl;
+          X[i] = upperx ? xh : xl;
         }
-        c = g.add(R, R, Y);
-        rtop += (int64_t)c;
       }
     }
-    wsync();
-    /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */
-    if (g.gl == 0) {
-      Tl[pos] = d << 1;
-      Tl[pos + 1] |= d >> 31;
+    /* phase-1 step s (digit n-2-s): the word entering the half at its bottom is X word
+     * NH-1-s; with U a multiple of KH its index in the lane is fixed by s mod U */
+    auto step1 = [&](int s0, int u) __attribute__((always_inline)) {
+      const int e = NH - 1 - s0 - u;
+      const uint32_t in0 = g.bcast(X[(KH - 1 - (u % KH)) % KH], e / KH);
+      sqrt_step<KH, K, L>(g, lds, n - 2 - s0 - u, W, in0, rtop, dtop, v);
+    };
+    static_assert(U % KH == 0, "phase-1 entering words need fixed lane indices");
+    int s0 = 0;
+    for (; s0 + U <= NH - 1; s0 += U) repeat<U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
+    repeat<(NH - 1) % U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
+    /* frame word q >= NH is W word q - NH; word NH-1 is X word 0 (the last radicand word
+     * below the half); words below it are zero */
+    {
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        const int q = g.gl * K + i;
+        const int sl = (q - NH) / KH;
+        const uint32_t w = g.bcast(W[i % KH], sl < 0 ? 0 : sl);
+        const uint32_t x0 = g.bcast(X[0], 0);
+        R[i] = q >= NH ? w : q == NH - 1 ? x0 : 0u;
+      }
     }
-    wsync();
-  };
-  /* digits n-2 down to 0, U per trip (see div) */
-  constexpr int U = div_steps_per_trip(K);
-  int p0 = n - 2;
-  for (; p0 + 1 >= U; p0 -= U) repeat<U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
-  repeat<(n - 1) % U>([&](int u) __attribute__((always_inline)) { step(p0 - u); });
+    int p1 = NH - 1;
+    for (; p1 + 1 >= U; p1 -= U)
+      repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
+    repeat<NH % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
+  } else {
+    /* digits n-2 down to 0, U per trip (see div) */
+    int p0 = n - 2;
+    for (; p0 + 1 >= U; p0 -= U)
+      repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p0 - u, R, 0u, rtop, dtop, v); });
+    repeat<(n - 1) % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p0 - u, R, 0u, rtop, dtop, v); });
+  }
   /* root S = (dtop : T) / 2 (n words), remainder (rtop : R) = X - S^2 */
   uint32_t S[K], D[K];
   {
@@ -962,7 +1034,7 @@ __device__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t e
 }
 
 template <int K, int L>
-__device__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   core_sqrt<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);

@@ -113,6 +113,31 @@ def test_adversarial_limbs(op, p):
         check(op, p, ae, al, None if op == "sqrt" else be, None if op == "sqrt" else bl, mode=mode, tag="adversarial")
 
 
+# A radicand from the reference's own generator (tests_generate_sqrt, 1024 bits) whose root
+# has a long run of all-ones words after 18 leading words and needs an add-back at digit 16
+# (the middle of the 32 digits); one compiled shape once returned a wrong lower half for it.
+SQRT_RUN_OF_ONES = int(
+    "9000000180000001C000000100000001000000003FFFFFFF7FFFFFFEBFFFFFFE7FFFFFFC7FFFFFFE7FFFFFFE"
+    "BFFFFFFF7FFFFFFE40000000BFFFFFFFC0000002C000000080000001800000000000000140000000000000"
+    "00BFFFFFFEFFFFFFFF800000007FFFFFFFBFFFFFFFFFFFFFFF80000000000000003FFFFFFFFFFFFFFF", 16)
+
+
+def test_sqrt_run_of_ones_all_variants():
+    p = 1024
+    n = p // 32
+    limbs = np.array([(SQRT_RUN_OF_ONES >> (32 * i)) & 0xFFFFFFFF for i in range(n)], np.uint32)
+    ae = np.array([2 * (O.BIAS - 164), 2 * (O.BIAS - 163)], np.uint32)   # even and odd exponent
+    al = np.tile(limbs, 2)
+    default = F.get_variant("sqrt", p)
+    try:
+        for v, (k, l) in enumerate(F.variants("sqrt", p)):
+            F.set_variant("sqrt", p, v)
+            for mode in ("rne", "rz", "ru"):
+                check("sqrt", p, ae, al, None, None, mode=mode, tag=f"run of ones K={k} L={l}")
+    finally:
+        F.set_variant("sqrt", p, default)
+
+
 @pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
 def test_special_classes(op):
     """All 9x9 class combinations (codes 2..8 and +-values)."""
