@@ -67,6 +67,18 @@ __device__ __forceinline__ uint32_t opaque_zero() {
   return z;
 }
 
+/* Hides the chunk's words from the compiler after a carry chain whose carry out is
+ * dropped.  Without it the compiler can fold that chain's last carry into the next chain
+ * on the same words (x1 = (a + c1) mod 2^32, then x1 + 0 + c becomes one a + c1 + c): the
+ * sum is right but that add's carry out also counts the dropped carry, and a chunk whose
+ * words were all ones then reports a carry it did not make.  (Seen on gfx950: an add-back
+ * after a resolved carry in sqrt.)  Emits no instructions. */
+template <int K>
+__device__ __forceinline__ void opaque_chunk(uint32_t (&x)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; i++) asm volatile("" : "+v"(x[i]));
+}
+
 /* r = a + b + cin (cin 0 or 1), returns the carry out of the chunk */
 template <int K>
 __device__ __forceinline__ uint32_t chunk_add(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t (&b)[K],
@@ -338,6 +350,7 @@ struct Group {
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(r) == 0xFFFFFFFFu, cin, cout);
     chunk_word_wave<K, false>(r, ci);
+    opaque_chunk(r);
     return cout;
   }
 
@@ -349,6 +362,7 @@ struct Group {
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(r) == 0u, bin, bout);
     chunk_word_wave<K, true>(r, bi);
+    opaque_chunk(r);
     return bout;
   }
 
@@ -364,6 +378,7 @@ struct Group {
     uint32_t cout;
     uint32_t ci = resolve(c != 0, and_all(x) == 0xFFFFFFFFu, 0u, cout);
     if (__ballot(ci != 0) != 0) chunk_word_wave<K, false>(x, ci);
+    opaque_chunk(x);
     return cout;
   }
 
@@ -377,6 +392,7 @@ struct Group {
     uint32_t bout;
     uint32_t bi = resolve(c != 0, or_all(x) == 0u, 0u, bout);
     if (__ballot(bi != 0) != 0) chunk_word_wave<K, true>(x, bi);
+    opaque_chunk(x);
     return bout;
   }
 

@@ -822,12 +822,10 @@ __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g,
 
 /* -------------------------------- sqrt -------------------------------- */
 /* shapes whose first n/2 - 1 digit steps run on the frame's top half (K/2 words per lane):
- * the steps per loop trip must be a multiple of K/2.  K = 4 (two words per lane in the
- * half) is left on the whole-frame loop: built that way, 4x8 gave a wrong root for one
- * input of the reference's generator at 1024 bits (tests/test_gpu_parity.py pins it) and
- * the cause was not found. */
+ * the steps per loop trip must be a multiple of K/2, and the top lane must still hold two
+ * words */
 __host__ __device__ constexpr bool sqrt_windowed(int K) {
-  return K >= 8 && K % 2 == 0 && div_steps_per_trip(K) % (K / 2) == 0;
+  return K >= 4 && K % 2 == 0 && div_steps_per_trip(K) % (K / 2) == 0;
 }
 
 /* One root digit (position pos) on R, KC words per lane holding frame words
