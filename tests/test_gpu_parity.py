@@ -113,6 +113,22 @@ def test_adversarial_limbs(op, p):
         check(op, p, ae, al, None if op == "sqrt" else be, None if op == "sqrt" else bl, mode=mode, tag="adversarial")
 
 
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("p", [512, 1024, 2048, 4096, 8192])
+def test_adversarial_limbs_all_variants(op, p):
+    """Every compiled shape on the reference generator's limb alphabet: long runs of all-ones
+    and all-zeros words drive carries through whole lane chunks and the add-back paths."""
+    ae, al, be, bl = adversarial(op, p, 512 if p <= 2048 else 128, 11 + p)
+    default = F.get_variant(op, p)
+    try:
+        for v, (k, l) in enumerate(F.variants(op, p)):
+            F.set_variant(op, p, v)
+            check(op, p, ae, al, None if op == "sqrt" else be, None if op == "sqrt" else bl,
+                  tag=f"adversarial K={k} L={l}")
+    finally:
+        F.set_variant(op, p, default)
+
+
 # A radicand from the reference's own generator (tests_generate_sqrt, 1024 bits) whose root
 # has a long run of all-ones words after 18 leading words and needs an add-back at digit 16
 # (the middle of the 32 digits); one compiled shape once returned a wrong lower half for it.
