@@ -106,6 +106,11 @@ def time_op(F, hip, op, p, count, steps, warmup, kind, seed, check_sample):
     prods, byts = algorithmic(op, p)
     res["tproducts_per_s"] = prods * res["ops_per_s"] / 1e12
     res["gb_per_s"] = byts * res["ops_per_s"] / 1e9
+    # the path's own roofline: INT32 multiply issue for mul/div/sqrt, HBM for add/sub
+    if prods:
+        res["frac_of_int32_mad_peak"] = res["tproducts_per_s"] / MAD_PEAK_T
+    else:
+        res["frac_of_hbm_peak"] = res["gb_per_s"] / HBM_GBS
     sample_in = (ae[sample], al[sample.start * n:sample.stop * n],
                  None if be is None else be[sample], None if bl is None else bl[sample.start * n:sample.stop * n])
     return res, sample_in, (re[sample], rl[sample.start * n:sample.stop * n])
