@@ -886,6 +886,63 @@ __device__ __forceinline__ void sqrt_step(const Group<L> &g, uint32_t *Tl, int p
   wsync();
 }
 
+/* sqrt_step with T's words [B, n) in registers (TC: KC words per lane, laid out like R)
+ * instead of LDS.  The digit's word pos sits at index idx (a constant of the unrolled step)
+ * of group lane (pos - B) / KC; the step touches no LDS, so it needs no wave syncs. */
+template <int KC, int K, int L>
+__device__ __forceinline__ void sqrt_step_regs(const Group<L> &g, int pos, int idx, uint32_t (&R)[KC],
+                                               uint32_t (&TC)[KC], uint32_t in0, int64_t &rtop, uint32_t dtop,
+                                               uint32_t &v) {
+  constexpr int n = K * L, B = n - KC * L;
+  const int owner = (pos - B) / KC;
+  const bool mine = g.gl == owner;
+  const uint32_t rt = R[KC - 1];
+  const uint32_t in = g.from_prev(R[KC - 1], in0);
+#pragma unroll
+  for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
+  R[0] = in;
+  rtop = (rtop << 32) | rt;
+  /* the top lane holds R's and T's two leading words */
+  const uint32_t r1 = R[KC - 1], r2 = R[KC - 2];
+  const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
+  const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
+  const uint32_t u0 = (r1 << 31) | (r2 >> 1);
+  const uint32_t t1 = TC[KC - 1], t2 = TC[KC - 2];
+  const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
+  if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
+  uint32_t d;
+  if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
+  else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
+  d = g.bcast(d, L - 1);
+  uint32_t Tr[KC];
+#pragma unroll
+  for (int i = 0; i < KC; i++) Tr[i] = (i == idx && mine) ? d : TC[i];
+  msub_step<KC, L>(g, R, rtop, d, Tr, dtop);
+  if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
+    for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
+      uint32_t c = g.add(R, R, Tr);
+      rtop += (int64_t)c + dtop;
+      d -= 1u;
+      uint32_t Y[KC];
+#pragma unroll
+      for (int i = 0; i < KC; i++) {
+        const bool here = i == idx && mine;
+        Y[i] = here ? d : 0u;
+        Tr[i] = here ? d : Tr[i];
+      }
+      c = g.add(R, R, Y);
+      rtop += (int64_t)c;
+    }
+  }
+  /* T <- 2S with the new digit: word pos = 2d mod 2^32, its carry sets bit 0 of pos+1 */
+  const bool next_mine = idx + 1 < KC ? mine : g.gl == owner + 1;
+#pragma unroll
+  for (int i = 0; i < KC; i++) {
+    if (i == idx) TC[i] = mine ? d << 1 : TC[i];
+    if (i == (idx + 1) % KC) TC[i] |= next_mine ? d >> 31 : 0u;
+  }
+}
+
 /* lds: n+1 words: T = 2S (+ current digit) at [0, n+1).  The root is T/2 at the end (the
  * bit above T's n words is dtop), so no separate root array is kept: a smaller LDS
  * footprint lets more waves share a CU. */
@@ -973,15 +1030,22 @@ __device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int 
     }
     /* phase-1 step s (digit n-2-s): the word entering the half at its bottom is X word
      * NH-1-s; with U a multiple of KH its index in the lane is fixed by s mod U */
+    /* T = 2S over the half stays in registers (TW, laid out like W) while these digits
+     * run: digit n-2-s lands at index (NH-2-s) mod KH, fixed by s mod U like X's */
+    uint32_t TW[KH];
+    lds_load(TW, Tl + NH + g.gl * KH);
     auto step1 = [&](int s0, int u) __attribute__((always_inline)) {
       const int e = NH - 1 - s0 - u;
       const uint32_t in0 = g.bcast(X[(KH - 1 - (u % KH)) % KH], e / KH);
-      sqrt_step<KH, K, L>(g, lds, n - 2 - s0 - u, W, in0, rtop, dtop, v);
+      const int idx = ((NH - 2 - u) % KH + KH) % KH;
+      sqrt_step_regs<KH, K, L>(g, n - 2 - s0 - u, idx, W, TW, in0, rtop, dtop, v);
     };
     static_assert(U % KH == 0, "phase-1 entering words need fixed lane indices");
     int s0 = 0;
     for (; s0 + U <= NH - 1; s0 += U) repeat<U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
     repeat<(NH - 1) % U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
+    lds_store(Tl + NH + g.gl * KH, TW);
+    wsync();
     /* frame word q >= NH is W word q - NH; word NH-1 is X word 0 (the last radicand word
      * below the half); words below it are zero */
     {
