@@ -1059,9 +1059,24 @@ __device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int 
       }
     }
     int p1 = NH - 1;
-    for (; p1 + 1 >= U; p1 -= U)
-      repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
-    repeat<NH % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
+    if constexpr (U % K == 0) {
+      /* K divides the steps per trip: digit pos lands at index (NH-1-u) mod K, so T stays in
+       * registers here too */
+      uint32_t TC[K];
+      lds_load(TC, Tl + g.gl * K);
+      auto step2 = [&](int u) __attribute__((always_inline)) {
+        sqrt_step_regs<K, K, L>(g, p1 - u, ((NH - 1 - u) % K + K) % K, R, TC, 0u, rtop, dtop, v);
+      };
+      for (; p1 + 1 >= U; p1 -= U) repeat<U>(step2);
+      repeat<NH % U>(step2);
+      wsync();
+      lds_store(Tl + g.gl * K, TC);
+      wsync();
+    } else {
+      for (; p1 + 1 >= U; p1 -= U)
+        repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
+      repeat<NH % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
+    }
   } else {
     /* digits n-2 down to 0, U per trip (see div) */
     int p0 = n - 2;
