@@ -1059,16 +1059,18 @@ __device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int 
       }
     }
     int p1 = NH - 1;
-    if constexpr (U % K == 0) {
-      /* K divides the steps per trip: digit pos lands at index (NH-1-u) mod K, so T stays in
-       * registers here too */
+    /* whole-frame digits take K steps per trip when K = 32 (16 otherwise): with K dividing
+     * the steps per trip, digit pos lands at index (NH-1-u) mod K, so T stays in registers
+     * here too */
+    constexpr int U2 = K == 32 ? 32 : U;
+    if constexpr (U2 % K == 0) {
       uint32_t TC[K];
       lds_load(TC, Tl + g.gl * K);
       auto step2 = [&](int u) __attribute__((always_inline)) {
         sqrt_step_regs<K, K, L>(g, p1 - u, ((NH - 1 - u) % K + K) % K, R, TC, 0u, rtop, dtop, v);
       };
-      for (; p1 + 1 >= U; p1 -= U) repeat<U>(step2);
-      repeat<NH % U>(step2);
+      for (; p1 + 1 >= U2; p1 -= U2) repeat<U2>(step2);
+      repeat<NH % U2>(step2);
       wsync();
       lds_store(Tl + g.gl * K, TC);
       wsync();
