@@ -299,8 +299,11 @@ __device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> 
  * r) by its own chunk of b into a window of 2*KD column sums; group lane 0's low KD
  * columns are complete and go to LDS; windows then shift down one lane.  After the last
  * round each lane holds KD high columns.  Columns are normalised to digits (local carry
- * chains, then the multi-bit carries between chunks, then a 1-bit ballot resolution),
- * stored to LDS, and the mantissa, round word and sticky are read back as 32-bit words.
+ * chains, then the multi-bit carries between chunks, then a 1-bit ballot resolution).
+ * The sticky bit comes from the low digits in registers.  The mantissa words and round
+ * word are funnel-shifted out of each lane's high digits and a few digits of its
+ * predecessor lane, in registers, where the shape allows (mul_extract_nb); otherwise the
+ * digits go to LDS and the words are read back from there.
  */
 constexpr int MUL_RADIX = 27;
 
@@ -324,6 +327,23 @@ __host__ __device__ constexpr int mul_lds_words(int K, int L) {
          : 2 * L * mul_digits_per_lane(K) > mul_adig_offset(K, L) + L * mul_digits_padded(K)
              ? 2 * L * mul_digits_per_lane(K)
              : mul_adig_offset(K, L) + L * mul_digits_padded(K);
+}
+
+/* 32K - 27*KD <= 0: how far a lane's K mantissa words run short of its KD digits */
+__host__ __device__ constexpr int mul_delta(int K) { return 32 * K - MUL_RADIX * mul_digits_per_lane(K); }
+/* Mantissa extraction in registers: lane gl's first mantissa bit lies
+ * s = 27*NB + delta*(L + gl) - (1 - full) bits above the start of the string made of the
+ * predecessor lane's NB top digits and its own KD high digits.  NB is the fewest digits
+ * (at most KD) for which every lane's s >= 0, lane 0 has the 32-bit round word below its
+ * first bit inside the string (s >= 32) and all s fit one two-word window from a
+ * 32-bit-aligned offset; 0 when no NB qualifies (the digits then go through LDS). */
+__host__ __device__ constexpr int mul_extract_nb(int K, int L) {
+  const int d = mul_delta(K), KD = mul_digits_per_lane(K);
+  for (int nb = 1; nb <= KD; nb++) {
+    const int smin = MUL_RADIX * nb + d * (2 * L - 1) - 1, smax = MUL_RADIX * nb + d * L;
+    if (smin >= 0 && MUL_RADIX * nb + d * L - 1 >= 32 && smax - smin / 32 * 32 < 64) return nb;
+  }
+  return 0;
 }
 
 /* x of lane+1 within this lane's 16-lane row (DPP row_shl:1); the row's lane 15 gets 0 */
@@ -633,21 +653,68 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
       add_carry_digits<KD>(dh, ch1);
     }
   }
-  /* digits to LDS: [0, ND) low, [ND, 2ND) high, then two zero words */
-#pragma unroll
-  for (int i = 0; i < KD; i++) { lds[g.gl * KD + i] = dl[i]; lds[ND + g.gl * KD + i] = dh[i]; }
-  if (g.gl == L - 1) { lds[2 * ND] = 0; lds[2 * ND + 1] = 0; }
-  wsync();
-
   /* P = A*B < 2^(2p); mantissa = top p bits from bit `base`, round word below, sticky */
   constexpr int p = 32 * n;
-  const int topdig = (2 * p - 1) / MUL_RADIX, topoff = (2 * p - 1) % MUL_RADIX;
-  const bool full = (lds[topdig] >> topoff) & 1u;
-  const int base = full ? p : p - 1;
-  uint32_t H[K];
+  constexpr int topdig = (2 * p - 1) / MUL_RADIX, topoff = (2 * p - 1) % MUL_RADIX;
+  bool full;
+  uint32_t H[K], rw;
+  if constexpr (mul_extract_nb(K, L) > 0) {
+    /* In registers: lane gl's mantissa words start at bit base + 32K*gl, at most
+     * 27*NB - 1 bits below its high digits (27*KD bits from B = 27*(ND + KD*gl)), so the
+     * lane packs its high digits and the NB digits below them (the predecessor lane's top
+     * high digits; for lane 0 the low half's top digits) into 32-bit words and funnel-
+     * shifts them. */
+    constexpr int NB = mul_extract_nb(K, L), DL = mul_delta(K), NDG = NB + KD;
+    constexpr int SMIN = MUL_RADIX * NB - 1 + DL * (2 * L - 1), OFF = SMIN / 32 * 32;
+    constexpr int hd = topdig - ND;
+    full = (g.bcast(dh[hd % KD], hd / KD) >> topoff) & 1u;
+    uint32_t Dg[NDG];
+    const int pred = (g.gl + L - 1) % L;
 #pragma unroll
-  for (int i = 0; i < K; i++) H[i] = digits_bits(lds, base + 32 * (g.gl * K + i));
-  uint32_t rw = digits_bits(lds, base - 32);
+    for (int m = 0; m < NB; m++) {
+      const uint32_t ex = g.gl == L - 1 ? dl[KD - NB + m] : dh[KD - NB + m];
+      Dg[m] = g.bcast(ex, pred);
+    }
+#pragma unroll
+    for (int k = 0; k < KD; k++) Dg[NB + k] = dh[k];
+    /* dense[w + 1] = bits [OFF + 32w, +32) of the string Dg (27 bits per digit), w = -1..K+1 */
+    uint32_t dense[K + 3];
+#pragma unroll
+    for (int w = -1; w <= K + 1; w++) {
+      const int t = OFF + 32 * w;
+      uint32_t x = 0;
+      if (t >= 0) {
+        const int m = t / MUL_RADIX, sh = t % MUL_RADIX;
+        if (m < NDG) x = Dg[m] >> sh;
+        if (m + 1 < NDG) x |= Dg[m + 1] << (MUL_RADIX - sh);
+        if (m + 2 < NDG && 2 * MUL_RADIX - sh < 32) x |= Dg[m + 2] << (2 * MUL_RADIX - sh);
+      }
+      dense[w + 1] = x;
+    }
+    /* this lane's first mantissa bit, relative to bit OFF of the string: 0 <= st < 64 */
+    const int st = MUL_RADIX * NB - (full ? 0 : 1) + DL * (L + g.gl) - OFF;
+    const bool j0 = st >= 32;
+    const uint32_t sh = (uint32_t)st & 31u;
+    uint32_t sel[K + 2];
+#pragma unroll
+    for (int w = 0; w < K + 2; w++) sel[w] = j0 ? dense[w + 1] : dense[w];
+#pragma unroll
+    for (int i = 0; i < K; i++) H[i] = __builtin_amdgcn_alignbit(sel[i + 2], sel[i + 1], sh);
+    /* the round word: the 32 bits below group lane 0's first mantissa bit */
+    rw = g.bcast(__builtin_amdgcn_alignbit(sel[1], sel[0], sh), 0);
+  } else {
+    /* digits to LDS: [0, ND) low, [ND, 2ND) high, then two zero words */
+#pragma unroll
+    for (int i = 0; i < KD; i++) { lds[g.gl * KD + i] = dl[i]; lds[ND + g.gl * KD + i] = dh[i]; }
+    if (g.gl == L - 1) { lds[2 * ND] = 0; lds[2 * ND + 1] = 0; }
+    wsync();
+    full = (lds[topdig] >> topoff) & 1u;
+    const int base0 = full ? p : p - 1;
+#pragma unroll
+    for (int i = 0; i < K; i++) H[i] = digits_bits(lds, base0 + 32 * (g.gl * K + i));
+    rw = digits_bits(lds, base0 - 32);
+  }
+  const int base = full ? p : p - 1;
   uint32_t loc;
   {
     /* sticky covers bits [0, lim), lim = base - 32 in {p - 33, p - 32}: low digits below
