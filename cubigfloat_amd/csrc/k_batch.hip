@@ -36,7 +36,9 @@ __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
     uint32_t X[K], Y[K], P[K], pe;
     load_chunk(X, d.x_limbs + k * n + (size_t)g.gl * K);
     load_chunk(Y, d.y_limbs + k * n + (size_t)g.gl * K);
-    core_mul<K, L, false>(g, lds, SW, d.mode, d.x_exps[k], X, d.y_exps[k], Y, pe, P);
+    /* A group of 16 lanes is one DPP row: the lockstep window shift reads no other group,
+     * so it stays exact while the wave's groups run chains of different lengths. */
+    core_mul<K, L, L == 16>(g, lds, SW, d.mode, d.x_exps[k], X, d.y_exps[k], Y, pe, P);
     uint32_t se, S[K];
     core_addsub<K, L>(g, lds, d.mode, false, acc_e, acc, pe, P, se, S);
     acc_e = se;
