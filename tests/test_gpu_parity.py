@@ -12,6 +12,8 @@ import pytest
 import cubigfloat_amd as F
 from oracle import oracle as O
 
+import mul_boundaries as MB
+
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -127,6 +129,22 @@ def test_adversarial_limbs_all_variants(op, p):
                   tag=f"adversarial K={k} L={l}")
     finally:
         F.set_variant(op, p, default)
+
+
+@pytest.mark.parametrize("p", F.PRECISIONS)
+def test_mul_rounding_boundaries_all_variants(p):
+    """Every compiled mul shape in every rounding mode on products that sit exactly on,
+    just below and just above the rounding boundaries, with the product's top bit clear and
+    set: these decide the mantissa's start, the round word and the sticky bit."""
+    ae, al, be, bl = MB.operands(p, 6, 101 + p)
+    default = F.get_variant("mul", p)
+    try:
+        for v, (k, l) in enumerate(F.variants("mul", p)):
+            F.set_variant("mul", p, v)
+            for mode in ("rne", "rna", "rz", "ru", "rd"):
+                check("mul", p, ae, al, be, bl, mode=mode, tag=f"boundaries K={k} L={l}")
+    finally:
+        F.set_variant("mul", p, default)
 
 
 # A radicand from the reference's own generator (tests_generate_sqrt, 1024 bits) whose root

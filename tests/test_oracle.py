@@ -6,7 +6,10 @@
   near-cancellation and special-class inputs in the four MPFR rounding modes;
 * ties-away-from-zero, which MPFR lacks (tests.c:69-71): brute force with Python
   integers at small precisions;
-* big-integer kernels (Knuth D, integer square root) against Python integers.
+* big-integer kernels (Knuth D, integer square root) against Python integers;
+* mul products on every rounding boundary (exact, ties with the kept part even and odd,
+  just below and above half; product's top bit clear and set) in all five modes, against
+  rounding by definition on the exact product with Python integers, at every precision.
 """
 import glob
 import os
@@ -16,6 +19,8 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+
+import mul_boundaries as MB
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OPS = ["add", "sub", "mul", "div", "sqrt"]
@@ -240,3 +245,13 @@ def test_oracle_dot_vs_mpfr(p):
         oe, ol = O.dot(p, lengths, xe, xl, ye, yl, mode=mode)
         me, ml = O.mpfr_dot(p, lengths, xe, xl, ye, yl, mode=mode)
         assert O.results_equal(oe, ol, me, ml, p).all()
+
+
+@pytest.mark.parametrize("p", [256, 512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 16384])
+def test_oracle_mul_rounding_boundaries(p):
+    ae, al, be, bl = MB.operands(p, 6, 101 + p)
+    for mode in MB.MODES:
+        oe, ol = O.op("mul", p, ae, al, be, bl, mode=mode)
+        xe, xl = MB.reference(p, ae, al, be, bl, mode)
+        eq = O.results_equal(oe, ol, xe, xl, p)
+        assert eq.all(), f"mode={mode}: {int((~eq).sum())} of {len(ae)} differ"
