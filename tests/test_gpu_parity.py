@@ -12,6 +12,7 @@ import pytest
 import cubigfloat_amd as F
 from oracle import oracle as O
 
+import divsqrt_boundaries as DB
 import mul_boundaries as MB
 
 pytestmark = pytest.mark.gpu
@@ -145,6 +146,25 @@ def test_mul_rounding_boundaries_all_variants(p):
                 check("mul", p, ae, al, be, bl, mode=mode, tag=f"boundaries K={k} L={l}")
     finally:
         F.set_variant("mul", p, default)
+
+
+@pytest.mark.parametrize("op", ["div", "sqrt"])
+@pytest.mark.parametrize("p", F.PRECISIONS)
+def test_div_sqrt_exact_boundaries_all_variants(op, p):
+    """Every compiled div and sqrt shape in every rounding mode on exact results (zero
+    remainder) and on operands one unit in the last place away from them."""
+    if op == "div":
+        ae, al, be, bl = DB.div_operands(p, 48, 7 + p)
+    else:
+        (ae, al), be, bl = DB.sqrt_operands(p, 48, 9 + p), None, None
+    default = F.get_variant(op, p)
+    try:
+        for v, (k, l) in enumerate(F.variants(op, p)):
+            F.set_variant(op, p, v)
+            for mode in DB.MODES:
+                check(op, p, ae, al, be, bl, mode=mode, tag=f"exact boundaries K={k} L={l}")
+    finally:
+        F.set_variant(op, p, default)
 
 
 # A radicand from the reference's own generator (tests_generate_sqrt, 1024 bits) whose root

@@ -9,7 +9,10 @@
 * big-integer kernels (Knuth D, integer square root) against Python integers;
 * mul products on every rounding boundary (exact, ties with the kept part even and odd,
   just below and above half; product's top bit clear and set) in all five modes, against
-  rounding by definition on the exact product with Python integers, at every precision.
+  rounding by definition on the exact product with Python integers, at every precision;
+* div and sqrt on exact results (a = u*v over v, squares of short integers) and one unit in
+  the last place away from them, in all five modes, against rounding by definition with
+  Python integers (long division, integer square root).
 """
 import glob
 import os
@@ -20,6 +23,7 @@ import pytest
 
 from oracle import oracle as O
 
+import divsqrt_boundaries as DB
 import mul_boundaries as MB
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -255,3 +259,16 @@ def test_oracle_mul_rounding_boundaries(p):
         xe, xl = MB.reference(p, ae, al, be, bl, mode)
         eq = O.results_equal(oe, ol, xe, xl, p)
         assert eq.all(), f"mode={mode}: {int((~eq).sum())} of {len(ae)} differ"
+
+
+@pytest.mark.parametrize("p", [256, 512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 16384])
+def test_oracle_div_sqrt_exact_boundaries(p):
+    ae, al, be, bl = DB.div_operands(p, 48, 7 + p)
+    xe, xl = DB.sqrt_operands(p, 48, 9 + p)
+    for mode in DB.MODES:
+        oe, ol = O.op("div", p, ae, al, be, bl, mode=mode)
+        re, rl = DB.div_reference(p, ae, al, be, bl, mode)
+        assert O.results_equal(oe, ol, re, rl, p).all(), f"div mode={mode}"
+        oe, ol = O.op("sqrt", p, xe, xl, None, None, mode=mode)
+        re, rl = DB.sqrt_reference(p, xe, xl, mode)
+        assert O.results_equal(oe, ol, re, rl, p).all(), f"sqrt mode={mode}"
