@@ -12,6 +12,7 @@ import pytest
 import cubigfloat_amd as F
 from oracle import oracle as O
 
+import addsub_boundaries as AB
 import divsqrt_boundaries as DB
 import mul_boundaries as MB
 
@@ -146,6 +147,22 @@ def test_mul_rounding_boundaries_all_variants(p):
                 check("mul", p, ae, al, be, bl, mode=mode, tag=f"boundaries K={k} L={l}")
     finally:
         F.set_variant("mul", p, default)
+
+
+@pytest.mark.parametrize("op", ["add", "sub"])
+@pytest.mark.parametrize("p", F.PRECISIONS)
+def test_add_sub_rounding_boundaries_all_variants(op, p):
+    """Every compiled add and sub shape in every rounding mode on sums whose discarded
+    part is exactly half, just below or above half, or zero, at exponent gaps 0..p+2."""
+    ae, al, be, bl = AB.operands(p, 96, 5 + p)
+    default = F.get_variant(op, p)
+    try:
+        for v, (k, l) in enumerate(F.variants(op, p)):
+            F.set_variant(op, p, v)
+            for mode in AB.MODES:
+                check(op, p, ae, al, be, bl, mode=mode, tag=f"boundaries K={k} L={l}")
+    finally:
+        F.set_variant(op, p, default)
 
 
 @pytest.mark.parametrize("op", ["div", "sqrt"])

@@ -12,7 +12,10 @@
   rounding by definition on the exact product with Python integers, at every precision;
 * div and sqrt on exact results (a = u*v over v, squares of short integers) and one unit in
   the last place away from them, in all five modes, against rounding by definition with
-  Python integers (long division, integer square root).
+  Python integers (long division, integer square root);
+* add and sub with the smaller operand's bits below the larger one's last place exactly
+  half, just below or above half, or zero, at exponent gaps 0..p+2 with both signs, in all
+  five modes, against rounding by definition of the exact sum.
 """
 import glob
 import os
@@ -23,6 +26,7 @@ import pytest
 
 from oracle import oracle as O
 
+import addsub_boundaries as AB
 import divsqrt_boundaries as DB
 import mul_boundaries as MB
 
@@ -272,3 +276,13 @@ def test_oracle_div_sqrt_exact_boundaries(p):
         oe, ol = O.op("sqrt", p, xe, xl, None, None, mode=mode)
         re, rl = DB.sqrt_reference(p, xe, xl, mode)
         assert O.results_equal(oe, ol, re, rl, p).all(), f"sqrt mode={mode}"
+
+
+@pytest.mark.parametrize("p", [256, 512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 16384])
+def test_oracle_add_sub_rounding_boundaries(p):
+    ae, al, be, bl = AB.operands(p, 96, 5 + p)
+    for op in ("add", "sub"):
+        for mode in AB.MODES:
+            oe, ol = O.op(op, p, ae, al, be, bl, mode=mode)
+            re, rl = AB.reference(op, p, ae, al, be, bl, mode)
+            assert O.results_equal(oe, ol, re, rl, p).all(), f"{op} mode={mode}"
