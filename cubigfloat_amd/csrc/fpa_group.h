@@ -296,6 +296,22 @@ struct Group {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((gbase + src) << 2, (int)x);
   }
 
+  /* x as held by the group's top lane.  Groups of 2 and 4 lanes sit inside one DPP quad, so
+   * the broadcast is one VALU move (quad_perm [1,1,3,3] or [3,3,3,3]) instead of a
+   * ds_bpermute round trip: it sits on the serial path of every div and sqrt digit step.
+   * Groups of 8 take each quad's lane 3, then the lower quad takes the upper quad's
+   * (row_shl:4; the row's last quad reads past the row, a value it does not use). */
+  __device__ __forceinline__ uint32_t bcast_top(uint32_t x) const {
+    if constexpr (L == 1) return x;
+    else if constexpr (L == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xF5, 0xF, 0xF, false);
+    else if constexpr (L == 4) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xFF, 0xF, 0xF, false);
+    else if constexpr (L == 8) {
+      const uint32_t t = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xFF, 0xF, 0xF, false);
+      const uint32_t u = (uint32_t)__builtin_amdgcn_mov_dpp((int)t, 0x104, 0xF, 0xF, true);
+      return gl < 4 ? u : t;
+    } else return bcast(x, L - 1);
+  }
+
   /* x of group lane gl-1; 'fill' in group lane 0 */
   __device__ __forceinline__ uint32_t from_prev(uint32_t x, uint32_t fill = 0) const {
     if constexpr (L == 1) return fill;

@@ -832,7 +832,7 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
     uint32_t q;
     if (u2 > d1 || (u2 == d1 && u1 >= d0)) q = 0xFFFFFFFFu;
     else q = fpa_div3by2(u2, u1, u0, d1, d0, v);
-    q = g.bcast(q, L - 1);
+    q = g.bcast_top(q);
     msub_step<K, L>(g, R, rtop, q, B, 0u);
     if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
       /* add back (the estimate exceeded the digit): rare, off the common path */
@@ -920,7 +920,7 @@ __device__ __forceinline__ void sqrt_step(const Group<L> &g, uint32_t *Tl, int p
   uint32_t d;
   if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
   else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
-  d = g.bcast(d, L - 1);
+  d = g.bcast_top(d);
   if (g.gl == 0) Tl[pos] = d;
   wsync();
   uint32_t Tr[KC];
@@ -980,7 +980,7 @@ __device__ __forceinline__ void sqrt_step_regs(const Group<L> &g, int pos, int i
   uint32_t d;
   if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
   else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
-  d = g.bcast(d, L - 1);
+  d = g.bcast_top(d);
   uint32_t Tr[KC];
 #pragma unroll
   for (int i = 0; i < KC; i++) Tr[i] = (i == idx && mine) ? d : TC[i];
