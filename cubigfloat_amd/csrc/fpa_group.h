@@ -312,6 +312,19 @@ struct Group {
     } else return bcast(x, L - 1);
   }
 
+  /* from_prev through a DPP row shift (row_shr:1) for groups inside one 16-lane row: a
+   * VALU move instead of a ds_bpermute round trip, for the digit steps' serial path.  Lane
+   * 0 of a group takes 'fill' (the row shift gave it the previous group's last lane, or
+   * zero at the row's start). */
+  __device__ __forceinline__ uint32_t from_prev_row(uint32_t x, uint32_t fill = 0) const {
+    if constexpr (L >= 2 && L <= 16) {
+      const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x111, 0xF, 0xF, true);
+      return gl == 0 ? fill : y;
+    } else {
+      return from_prev(x, fill);
+    }
+  }
+
   /* x of group lane gl-1; 'fill' in group lane 0 */
   __device__ __forceinline__ uint32_t from_prev(uint32_t x, uint32_t fill = 0) const {
     if constexpr (L == 1) return fill;

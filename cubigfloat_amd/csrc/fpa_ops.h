@@ -773,7 +773,7 @@ __device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], i
     bw = o;
   }
   uint32_t out = c + bw;                     /* <= 2^32-1: high product word is <= 2^32-2 */
-  uint32_t bout = g.sub_lane_words(R, g.from_prev(out));
+  uint32_t bout = g.sub_lane_words(R, g.from_prev_row(out));
   rtop = rtop - (int64_t)out - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
 }
 
@@ -822,7 +822,7 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
   auto step = [&](int j) __attribute__((always_inline)) {
     /* R <- R * 2^32; the word leaving the top becomes rtop (top lane only) */
     int64_t rtop = R[K - 1];
-    uint32_t in = g.from_prev(R[K - 1]);
+    uint32_t in = g.from_prev_row(R[K - 1]);
 #pragma unroll
     for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
     R[0] = in;
@@ -905,7 +905,7 @@ __device__ __forceinline__ void sqrt_step(const Group<L> &g, uint32_t *Tl, int p
                                           int64_t &rtop, uint32_t dtop, uint32_t &v) {
   constexpr int n = K * L, B = n - KC * L;
   const uint32_t rt = R[KC - 1];
-  const uint32_t in = g.from_prev(R[KC - 1], in0);
+  const uint32_t in = g.from_prev_row(R[KC - 1], in0);
 #pragma unroll
   for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
   R[0] = in;
@@ -964,7 +964,7 @@ __device__ __forceinline__ void sqrt_step_regs(const Group<L> &g, int pos, int i
   const int owner = (pos - B) / KC;
   const bool mine = g.gl == owner;
   const uint32_t rt = R[KC - 1];
-  const uint32_t in = g.from_prev(R[KC - 1], in0);
+  const uint32_t in = g.from_prev_row(R[KC - 1], in0);
 #pragma unroll
   for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
   R[0] = in;
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int 
   uint32_t dtop;
   {
     uint32_t rt = g.template word<K, n - 1>(R);
-    uint32_t in = g.from_prev(R[K - 1], shift_in);
+    uint32_t in = g.from_prev_row(R[K - 1], shift_in);
 #pragma unroll
     for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
     R[0] = in;
