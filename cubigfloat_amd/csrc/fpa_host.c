@@ -31,21 +31,39 @@
     }                                                                                      \
   } while (0)
 
-/* per-op launchers defined in k_<op>.hip */
-int fpa_impl_launch_neg(const fpa_arguments *a, int K, int L, void *stream);
-int fpa_impl_launch_add(const fpa_arguments *a, int K, int L, void *stream);
-int fpa_impl_launch_sub(const fpa_arguments *a, int K, int L, void *stream);
-int fpa_impl_launch_mul(const fpa_arguments *a, int K, int L, void *stream);
-int fpa_impl_launch_div(const fpa_arguments *a, int K, int L, void *stream);
-int fpa_impl_launch_sqrt(const fpa_arguments *a, int K, int L, void *stream);
+/* per-op launchers defined in k_<op>*.hip, one per compilation part: each launches the
+ * shapes its part compiles and returns -1 for the others */
+typedef int (*launch_fn)(const fpa_arguments *, int, int, void *);
+#define FPA_PART_DECL(name, i) int fpa_impl_launch_##name##_part##i(const fpa_arguments *a, int K, int L, void *stream);
+FPA_PART_DECL(neg, 0)
+FPA_PART_DECL(add, 0)
+FPA_PART_DECL(sub, 0)
+FPA_PART_DECL(mul, 0)
+FPA_PART_DECL(div, 0) FPA_PART_DECL(div, 1)
+FPA_PART_DECL(sqrt, 0) FPA_PART_DECL(sqrt, 1) FPA_PART_DECL(sqrt, 2)
+FPA_PART_DECL(sqrt, 3) FPA_PART_DECL(sqrt, 4) FPA_PART_DECL(sqrt, 5)
+#undef FPA_PART_DECL
 int fpa_impl_launch_sgn(uint32_t count, const uint32_t *x_exps, int32_t *r, void *stream);
 int fpa_impl_launch_setcmp(int op, const fpa_arguments *args, int K, int L, void *stream);
 
-typedef int (*launch_fn)(const fpa_arguments *, int, int, void *);
-static const launch_fn launchers[FPA_OP_COUNT] = {
-  fpa_impl_launch_neg, fpa_impl_launch_add, fpa_impl_launch_sub,
-  fpa_impl_launch_mul, fpa_impl_launch_div, fpa_impl_launch_sqrt,
+#define FPA_MAX_PARTS 6
+static const launch_fn launch_parts[FPA_OP_COUNT][FPA_MAX_PARTS] = {
+  { fpa_impl_launch_neg_part0 },
+  { fpa_impl_launch_add_part0 },
+  { fpa_impl_launch_sub_part0 },
+  { fpa_impl_launch_mul_part0 },
+  { fpa_impl_launch_div_part0, fpa_impl_launch_div_part1 },
+  { fpa_impl_launch_sqrt_part0, fpa_impl_launch_sqrt_part1, fpa_impl_launch_sqrt_part2,
+    fpa_impl_launch_sqrt_part3, fpa_impl_launch_sqrt_part4, fpa_impl_launch_sqrt_part5 },
 };
+
+static int launch_op(int op, const fpa_arguments *a, int K, int L, void *stream) {
+  for (int i = 0; i < FPA_MAX_PARTS && launch_parts[op][i]; i++) {
+    int rc = launch_parts[op][i](a, K, L, stream);
+    if (rc != -1) return rc;
+  }
+  return -1;
+}
 
 /* compiled shape table */
 typedef struct { int n, K, L; } shape_t;
@@ -78,9 +96,13 @@ static int prec_index(uint32_t precision) {
 
 int fpa_precision_supported(uint32_t precision) { return prec_index(precision) >= 0; }
 
-/* shapes an op may use: every compiled one, except that the dot kernel exists only for
- * K <= FPA_DOT_MAX_K */
-static int shape_allowed(int op, int s) { return op != FPA_KERNEL_DOT || shapes[s].K <= FPA_DOT_MAX_K; }
+/* shapes an op may use: every compiled one, except that mul exists only for
+ * K <= FPA_MUL_MAX_K and the dot kernel only for K <= FPA_DOT_MAX_K */
+static int shape_allowed(int op, int s) {
+  if (op == FPA_KERNEL_DOT) return shapes[s].K <= FPA_DOT_MAX_K;
+  if (op == FPA_OP_MUL) return shapes[s].K <= FPA_MUL_MAX_K;
+  return 1;
+}
 
 /* variant v of (op, limb count n) -> index into shapes[], or -1 */
 static int shape_of(int op, int n, int v) {
@@ -202,7 +224,7 @@ int fpa_kernel_launch(int op, const fpa_arguments *args, int variant, void *stre
   int s = prec_index(args->precision) < 0 ? -1 : shape_of(op, (int)(args->precision / 32), variant);
   if (s < 0) return -1;
   if (args->count == 0) return 0;
-  return launchers[op](args, shapes[s].K, shapes[s].L, stream);
+  return launch_op(op, args, shapes[s].K, shapes[s].L, stream);
 }
 
 int fpa_kernel_time(int op, const fpa_arguments *args, int variant, int reps, void *stream, float *ms) {

@@ -70,14 +70,61 @@ static int launch_shape(const fpa_arguments &a, hipStream_t st) {
 
 }  // namespace fpa
 
-/* defines extern "C" int fpa_impl_launch_<name>(const fpa_arguments*, int K, int L, void *stream) */
-#define FPA_DEFINE_LAUNCHER(name, OPV)                                                          \
-  extern "C" int fpa_impl_launch_##name(const fpa_arguments *a, int K, int L, void *stream) { \
-    hipStream_t st = (hipStream_t)stream;                                                       \
-    _Pragma("clang diagnostic push")                                                            \
-    FPA_SHAPES(FPA_LAUNCH_CASE_##name)                                                          \
-    _Pragma("clang diagnostic pop")                                                             \
-    return -1;                                                                                  \
+namespace fpa {
+
+/* The compiled shapes as data, in FPA_SHAPES order. */
+struct ShapeNKL { int n, K, L; };
+#define FPA_SHAPE_NKL(n_, K_, L_) ShapeNKL{n_, K_, L_},
+constexpr ShapeNKL kShapes[] = { FPA_SHAPES(FPA_SHAPE_NKL) };
+#undef FPA_SHAPE_NKL
+constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+
+/* Shapes an op is compiled for: mul stops at FPA_MUL_MAX_K limbs per lane. */
+constexpr bool op_has_shape(int op, int K) { return op != 3 || K <= FPA_MUL_MAX_K; }
+
+/* Compilation parts.  An op's shapes can be split over several files that build in
+ * parallel; shapes are dealt to parts heaviest first, each to the least-loaded part, by a
+ * rough compile-time weight (sqrt with 32 limbs per lane: about a minute per shape; with
+ * 16: about 20 s; div with 32: about 12 s; most others 4-10 s). */
+constexpr int compile_weight(int op, int K) {
+  return op == 5 ? (K == 32 ? 12 : K == 16 ? 3 : 1) : op == 4 ? (K == 32 ? 2 : 1) : 1;
+}
+constexpr int compile_part(int op, int K, int L, int nparts) {
+  int load[16] = {};
+  int part = -1;
+  for (int w = 12; w >= 1; w--) {
+    for (int i = 0; i < kNumShapes; i++) {
+      if (!op_has_shape(op, kShapes[i].K) || compile_weight(op, kShapes[i].K) != w) continue;
+      int best = 0;
+      for (int q = 1; q < nparts; q++)
+        if (load[q] < load[best]) best = q;
+      load[best] += w;
+      if (kShapes[i].K == K && kShapes[i].L == L) part = best;
+    }
+  }
+  return part;
+}
+
+/* Launch (K, L) if this part compiles it; -1 if it does not.  A template, so the shapes
+ * of other parts are never instantiated here. */
+template <int OP, int PART, int NPARTS>
+static int launch_part(const fpa_arguments *a, int K, int L, hipStream_t st) {
+#define FPA_LAUNCH_CASE_PART(n, K_, L_)                                   \
+  if constexpr (op_has_shape(OP, K_) && compile_part(OP, K_, L_, NPARTS) == PART) \
+    if (K == K_ && L == L_) return launch_shape<K_, L_, OP>(*a, st);
+  FPA_SHAPES(FPA_LAUNCH_CASE_PART)
+#undef FPA_LAUNCH_CASE_PART
+  return -1;
+}
+
+}  // namespace fpa
+
+/* defines extern "C" int fpa_impl_launch_<name>_part<PART>(const fpa_arguments*, int K,
+ * int L, void *stream): launches the shapes of compilation part PART of NPARTS, returns
+ * -1 for a shape of another part */
+#define FPA_DEFINE_LAUNCHER_PART(name, OPV, PART, NPARTS)                                            \
+  extern "C" int fpa_impl_launch_##name##_part##PART(const fpa_arguments *a, int K, int L, void *stream) { \
+    return fpa::launch_part<OPV, PART, NPARTS>(a, K, L, (hipStream_t)stream);                           \
   }
 
 #endif

@@ -64,10 +64,11 @@ __constant__ uint8_t k_mul_tab[81] = {
   FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN, FPA_C_SNAN,
 };
 /* classes MPFR represents follow MPFR: x/0, inf/0 -> +-INF; inf/x signed by the
- * quotient; x/inf -> ZERO.  TINY/SNAN entries are the reference's (fpa_warp_div.h:1-12). */
+ * quotient; x/inf -> ZERO.  Entries with a TINY or SNAN operand are the reference's
+ * (fpa_warp_div.h:1-12), among them value/TINY -> INF with the quotient's sign. */
 __constant__ uint8_t k_div_tab[81] = {
-  FPA_A_DIV,      FPA_A_DIV,      FPA_C_POS_INF, FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
-  FPA_A_DIV,      FPA_A_DIV,      FPA_C_NEG_INF, FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_DIV,      FPA_A_DIV,      FPA_C_POS_INF, FPA_C_NEG_INF,  FPA_C_POS_INF,  FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
+  FPA_A_DIV,      FPA_A_DIV,      FPA_C_NEG_INF, FPA_C_POS_INF,  FPA_C_NEG_INF,  FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
   FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_ZERO,     FPA_C_NAN, FPA_C_SNAN,
   FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NAN,     FPA_C_NAN,      FPA_C_NAN,      FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_NAN, FPA_C_SNAN,
   FPA_C_POS_TINY, FPA_C_NEG_TINY, FPA_C_NAN,     FPA_C_NAN,      FPA_C_NAN,      FPA_C_NEG_TINY, FPA_C_POS_TINY, FPA_C_NAN, FPA_C_SNAN,

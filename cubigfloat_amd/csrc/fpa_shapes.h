@@ -20,6 +20,11 @@
   X(256, 16, 16) X(256, 32, 8)  X(256, 8, 32)                \
   X(512, 16, 32) X(512, 32, 16)
 
+/* mul is compiled only for shapes with K <= FPA_MUL_MAX_K: with 28 or 32 limbs per lane its
+ * digit window no longer fits the registers, it spills, and it ran 20-40x slower than the
+ * other shapes of the same precision. */
+#define FPA_MUL_MAX_K 24
+
 /* The dot-chain kernel (one mul core and one add core per element) is compiled only for
  * shapes with K <= FPA_DOT_MAX_K: above that its add core no longer inlines and becomes a
  * real call, which no kernel of this library may contain. */

@@ -1,6 +1,4 @@
-/* Kernel instantiations for fpa_add (all compiled shapes). */
+/* Kernel instantiations for fpa_add (all its compiled shapes). */
 #include "fpa_kernel_common.h"
 
-#define FPA_LAUNCH_CASE_add(n, K_, L_) \
-  if (K == K_ && L == L_) return fpa::launch_shape<K_, L_, 1>(*a, st);
-FPA_DEFINE_LAUNCHER(add, 1)
+FPA_DEFINE_LAUNCHER_PART(add, 1, 0, 1)

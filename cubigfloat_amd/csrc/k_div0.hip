@@ -1,0 +1,4 @@
+/* Kernel instantiations for fpa_div, compilation part 0 of 2. */
+#include "fpa_kernel_common.h"
+
+FPA_DEFINE_LAUNCHER_PART(div, 4, 0, 2)

@@ -1,6 +1,4 @@
-/* Kernel instantiations for fpa_mul (all compiled shapes). */
+/* Kernel instantiations for fpa_mul (all its compiled shapes). */
 #include "fpa_kernel_common.h"
 
-#define FPA_LAUNCH_CASE_mul(n, K_, L_) \
-  if (K == K_ && L == L_) return fpa::launch_shape<K_, L_, 3>(*a, st);
-FPA_DEFINE_LAUNCHER(mul, 3)
+FPA_DEFINE_LAUNCHER_PART(mul, 3, 0, 1)

@@ -1,6 +1,4 @@
-/* Kernel instantiations for fpa_neg (all compiled shapes). */
+/* Kernel instantiations for fpa_neg (all its compiled shapes). */
 #include "fpa_kernel_common.h"
 
-#define FPA_LAUNCH_CASE_neg(n, K_, L_) \
-  if (K == K_ && L == L_) return fpa::launch_shape<K_, L_, 0>(*a, st);
-FPA_DEFINE_LAUNCHER(neg, 0)
+FPA_DEFINE_LAUNCHER_PART(neg, 0, 0, 1)

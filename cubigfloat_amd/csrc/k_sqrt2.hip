@@ -1,0 +1,4 @@
+/* Kernel instantiations for fpa_sqrt, compilation part 2 of 6. */
+#include "fpa_kernel_common.h"
+
+FPA_DEFINE_LAUNCHER_PART(sqrt, 5, 2, 6)

@@ -95,10 +95,11 @@ static const uint8_t mul_tab[81] = {
 /* Rows/columns MPFR can represent (values, zero, +-inf, nan) follow MPFR:
  * x/0 -> +-INF (reference: NAN), inf/0 -> +-INF (reference: NAN), inf/x has the sign
  * of the quotient (the reference's inf rows have it flipped), x/inf -> ZERO (MPFR +-0;
- * reference: TINY).  Entries with a TINY or SNAN operand are the reference's. */
+ * reference: TINY).  These are 12 entries.  Entries with a TINY or SNAN operand are the
+ * reference's, e.g. value/TINY -> INF with the quotient's sign (fpa_warp_div.h:4-5). */
 static const uint8_t div_tab[81] = {
-  A_DIV,   A_DIV,   C_PINF, C_NTINY, C_PTINY, C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
-  A_DIV,   A_DIV,   C_NINF, C_PTINY, C_NTINY, C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
+  A_DIV,   A_DIV,   C_PINF, C_NINF,  C_PINF,  C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
+  A_DIV,   A_DIV,   C_NINF, C_PINF,  C_NINF,  C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
   C_ZERO,  C_ZERO,  C_NAN,  C_ZERO,  C_ZERO,  C_ZERO,  C_ZERO,  C_NAN, C_SNAN,
   C_NTINY, C_PTINY, C_NAN,  C_NAN,   C_NAN,   C_PTINY, C_NTINY, C_NAN, C_SNAN,
   C_PTINY, C_NTINY, C_NAN,  C_NAN,   C_NAN,   C_NTINY, C_PTINY, C_NAN, C_SNAN,

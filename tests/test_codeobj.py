@@ -5,7 +5,11 @@ Every kernel body must be fully inlined: no kernel may call a device function.  
 whose cores were compiled as real calls (pointer arguments into the private segment,
 reached through flat loads/stores) are the only ones that ever failed on the device,
 so the build keeps them out: the dot-chain kernel is instantiated only for shapes whose
-body inlines, and this test fails if any call appears."""
+body inlines, and this test fails if any call appears.
+
+The element-wise kernels of one op may be split over several compilation units (parts
+that build in parallel): every (op, shape) the dispatcher can launch must be compiled
+exactly once, in one code object."""
 import os
 import re
 import shutil
@@ -56,3 +60,26 @@ def test_no_kernel_calls_a_function(tmp_path):
                 calls[cur] = calls.get(cur, 0) + 1
     assert kernels > 0
     assert not calls, f"kernels with device-function calls: {sorted(calls)[:8]}"
+
+
+def _shapes():
+    txt = open(os.path.join(ROOT, "cubigfloat_amd", "csrc", "fpa_shapes.h")).read()
+    shapes = [tuple(int(v) for v in m) for m in re.findall(r"X\((\d+), (\d+), (\d+)\)", txt)]
+    mul_max = int(re.search(r"#define FPA_MUL_MAX_K (\d+)", txt).group(1))
+    return shapes, mul_max
+
+
+@pytest.mark.skipif(not (os.path.exists(LIB) and shutil.which("objcopy") and os.path.exists(f"{LLVM}/llvm-readelf")),
+                    reason="library or LLVM tools missing")
+def test_every_shape_compiled_once(tmp_path):
+    seen = {}
+    for i, co in enumerate(code_objects(tmp_path)):
+        syms = subprocess.run([f"{LLVM}/llvm-readelf", "--dyn-syms", "-C", str(co)], check=True, capture_output=True,
+                              text=True).stdout
+        for m in set(re.findall(r"fpa::fpa_kernel<(\d+), (\d+), (\d+)>\(fpa_arguments\)", syms)):
+            key = tuple(int(v) for v in m)
+            assert key not in seen, f"kernel K={key[0]} L={key[1]} op={key[2]} in two code objects"
+            seen[key] = i
+    shapes, mul_max = _shapes()
+    want = {(K, L, op) for op in range(6) for (_, K, L) in shapes if op != 3 or K <= mul_max}
+    assert set(seen) == want, (sorted(want - set(seen))[:5], sorted(set(seen) - want)[:5])
