@@ -80,6 +80,8 @@ lib.fpa_scalar_div3by2.argtypes = [ctypes.c_uint32] * 6
 lib.fpa_scalar_div3by2.restype = ctypes.c_uint32
 lib.fpa_scalar_isqrt64.argtypes = [ctypes.c_uint64]
 lib.fpa_scalar_isqrt64.restype = ctypes.c_uint32
+lib.fpa_scalar_encode.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+lib.fpa_scalar_encode.restype = ctypes.c_uint32
 lib.fpa_workload_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
                                       ctypes.c_uint64, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p]
 
@@ -183,6 +185,27 @@ class DeviceInt32:
         if self.ptr:
             DeviceInt32._hip.hipFree(self.ptr)
             self.ptr = None
+
+
+SCALAR_SETTERS = {"set_ui": (lib.fpa_set_ui, np.uint64, ctypes.c_uint64),
+                  "set_si": (lib.fpa_set_si, np.int64, ctypes.c_int64),
+                  "set_float": (lib.fpa_set_float, np.float32, ctypes.c_float),
+                  "set_double": (lib.fpa_set_double, np.float64, ctypes.c_double)}
+
+
+def set_scalars(r, name, values):
+    """fpa_set_ui / fpa_set_si / fpa_set_float / fpa_set_double (name) of a host array."""
+    fn, dt, ct = SCALAR_SETTERS[name]
+    v = np.ascontiguousarray(values, dtype=dt)
+    assert v.size == r.count
+    return fn(r.ref, v.ctypes.data_as(ctypes.POINTER(ct)))
+
+
+def scalar_encode(kind, bits):
+    """(exponent word, left-aligned significand) of one fpa_set_* source (host helper)."""
+    top = ctypes.c_uint64(0)
+    w = lib.fpa_scalar_encode(kind, bits, ctypes.byref(top))
+    return w, top.value
 
 
 def set_precision(r, x, mode=ROUND_TIES_TO_EVEN):

@@ -306,6 +306,9 @@ fpa_result fpa_clear(fpa_t fpa) {
 
 fpa_result fpa_copy_array_to_device(fpa_t fpa, uint32_t *exps, uint32_t *limbs) {
   size_t words = (size_t)fpa->count * (fpa->precision / 32);
+  /* work queued on the fpa stream may still read these arrays; a stream created
+   * non-blocking does not wait for the null stream's copy by itself */
+  if (cur_stream) FPA_HIP(hipStreamSynchronize((hipStream_t)cur_stream));
   FPA_HIP(hipMemcpy(fpa->gpu_exps, exps, (size_t)fpa->count * 4, hipMemcpyHostToDevice));
   FPA_HIP(hipMemcpy(fpa->gpu_limbs, limbs, words * 4, hipMemcpyHostToDevice));
   return fpa_success;
@@ -355,71 +358,39 @@ fpa_result fpa_sqrt(fpa_t r, fpa_t x, fpa_rounding_mode m) { return run(FPA_OP_S
 
 /* ------------------------- conversions (exact) ------------------------- */
 
-/* host staging of |u| * 2^0 with sign into value i of (exps, limbs) */
-static void encode_u64(uint32_t *exp, uint32_t *limbs, int n, uint64_t u, uint32_t sign) {
-  memset(limbs, 0, 4 * (size_t)n);
-  if (u == 0) { *exp = FPA_C_ZERO; return; }
-  int bl = 64 - __builtin_clzll(u);
-  uint64_t m = u << (64 - bl);                     /* top bit set */
-  limbs[n - 1] = (uint32_t)(m >> 32);
-  limbs[n - 2] = (uint32_t)m;
-  *exp = (uint32_t)(2 * (FPA_BIAS_LL + bl) + sign);
+/* fpa_set_ui/si/float/double (fpa_dispatch.h:8-86): the 4- or 8-byte sources go to a
+ * staging buffer in device memory and one kernel writes every value's exponent word and
+ * limbs there (k_setcmp.hip), as the reference does (cudaMalloc, copy, launch,
+ * cudaFree).  The call waits for the kernel before it frees the staging buffer. */
+static fpa_result set_scalars(fpa_t r, const void *values, size_t elem, int kind) {
+  if (!fpa_precision_supported(r->precision)) return fpa_invalid_precisions;
+  if (r->count == 0) return fpa_success;
+  void *dv = NULL;
+  FPA_HIP(hipMalloc(&dv, elem * (size_t)r->count));
+  FPA_HIP(hipMemcpy(dv, values, elem * (size_t)r->count, hipMemcpyHostToDevice));
+  fpa_arguments a;
+  memset(&a, 0, sizeof a);
+  a.count = r->count;
+  a.precision = r->precision;
+  a.r_exps = r->gpu_exps;
+  a.r_limbs = r->gpu_limbs;
+  a.values = dv;
+  int K, L;
+  fpa_kernel_variant_shape(FPA_OP_NEG, r->precision, fpa_kernel_get_variant_for_count(FPA_OP_NEG, r->precision, r->count),
+                           &K, &L);
+  int rc = fpa_impl_launch_setcmp(2 + kind, &a, K, L, cur_stream);
+  if (rc != 0) { fprintf(stderr, "fpa_set: launch failed (%d)\n", rc); exit(1); }
+  FPA_HIP(hipStreamSynchronize((hipStream_t)cur_stream));
+  FPA_HIP(hipFree(dv));
+  return fpa_success;
 }
 
-/* value = frac * 2^e2 with 53-bit (or 24-bit) integer significand */
-static void encode_fp(uint32_t *exp, uint32_t *limbs, int n, double x) {
-  if (isnan(x)) { memset(limbs, 0, 4 * (size_t)n); *exp = FPA_C_NAN; return; }
-  if (isinf(x)) { memset(limbs, 0, 4 * (size_t)n); *exp = x < 0 ? FPA_C_NEG_INF : FPA_C_POS_INF; return; }
-  if (x == 0) { memset(limbs, 0, 4 * (size_t)n); *exp = FPA_C_ZERO; return; }
-  int e;
-  double f = frexp(fabs(x), &e);                   /* |x| = f * 2^e, f in [1/2, 1) */
-  uint64_t m = (uint64_t)ldexp(f, 64);             /* exact: at most 53 significant bits */
-  memset(limbs, 0, 4 * (size_t)n);
-  limbs[n - 1] = (uint32_t)(m >> 32);
-  limbs[n - 2] = (uint32_t)m;
-  *exp = (uint32_t)(2 * (FPA_BIAS_LL + e) + (x < 0));
-}
+fpa_result fpa_set_ui(fpa_t r, uint64_t *ui) { return set_scalars(r, ui, 8, FPA_SRC_U64); }
+fpa_result fpa_set_si(fpa_t r, int64_t *si) { return set_scalars(r, si, 8, FPA_SRC_S64); }
+fpa_result fpa_set_float(fpa_t r, float *floats) { return set_scalars(r, floats, 4, FPA_SRC_F32); }
+fpa_result fpa_set_double(fpa_t r, double *doubles) { return set_scalars(r, doubles, 8, FPA_SRC_F64); }
 
-static fpa_result upload_staged(fpa_t r, uint32_t *e, uint32_t *l) {
-  fpa_result rc = fpa_copy_array_to_device(r, e, l);
-  free(e);
-  free(l);
-  return rc;
-}
-
-#define STAGE(r)                                                                        \
-  int n_ = (int)((r)->precision / 32);                                                  \
-  uint32_t *se = (uint32_t *)malloc((size_t)(r)->count * 4 + 4);                        \
-  uint32_t *sl = (uint32_t *)malloc((size_t)(r)->count * n_ * 4 + 4);                   \
-  if (!se || !sl) { free(se); free(sl); return fpa_math_error; }
-
-fpa_result fpa_set_ui(fpa_t r, uint64_t *ui) {
-  STAGE(r);
-  for (uint32_t i = 0; i < r->count; i++) encode_u64(&se[i], sl + (size_t)i * n_, n_, ui[i], 0);
-  return upload_staged(r, se, sl);
-}
-
-fpa_result fpa_set_si(fpa_t r, int64_t *si) {
-  STAGE(r);
-  for (uint32_t i = 0; i < r->count; i++) {
-    int64_t v = si[i];
-    uint64_t mag = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
-    encode_u64(&se[i], sl + (size_t)i * n_, n_, mag, v < 0);
-  }
-  return upload_staged(r, se, sl);
-}
-
-fpa_result fpa_set_float(fpa_t r, float *floats) {
-  STAGE(r);
-  for (uint32_t i = 0; i < r->count; i++) encode_fp(&se[i], sl + (size_t)i * n_, n_, (double)floats[i]);
-  return upload_staged(r, se, sl);
-}
-
-fpa_result fpa_set_double(fpa_t r, double *doubles) {
-  STAGE(r);
-  for (uint32_t i = 0; i < r->count; i++) encode_fp(&se[i], sl + (size_t)i * n_, n_, doubles[i]);
-  return upload_staged(r, se, sl);
-}
+uint32_t fpa_scalar_encode(int kind, uint64_t bits, uint64_t *top) { return fpa_encode_scalar(kind, bits, top); }
 
 /* set / sgn / cmp (fpa_dispatch.h:88-153): the kernels in k_setcmp.hip, launched on the
  * fpa stream with the dispatcher's neg shape (a copy-like kernel) for the precision. */

@@ -11,6 +11,9 @@
  *                      DIV32/APPROX32, fpa_math.h:1-50)
  *   fpa_isqrt64     -- floor(sqrt(u)) for u < 2^64 (first root digit; the reference's
  *                      SQRT64, fpa_math.h:68-110, computes the same value bitwise)
+ *   fpa_encode_scalar -- exact fpa encoding of a 64-bit integer or an IEEE binary32 /
+ *                      binary64 value (fpa_set_ui/si/float/double, reference
+ *                      fpa_warp_set.h:1-132)
  *
  * Compiled by hipcc for the device and by gcc/g++ for the host tests.
  */
@@ -97,6 +100,50 @@ FPA_HD uint32_t fpa_isqrt64(uint64_t u) {
   while (r * r > u) r--;
   while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= u) r++;
   return (uint32_t)r;
+}
+
+/* Source kinds of fpa_encode_scalar (the four fpa_set_* conversions) */
+#define FPA_SRC_U64 0
+#define FPA_SRC_S64 1
+#define FPA_SRC_F32 2
+#define FPA_SRC_F64 3
+
+/* Exact fpa encoding of one source value: returns the exponent word, or a class code for
+ * zero (either sign), infinities and NaN.  For a value, *top receives its magnitude's
+ * significand left-aligned (bit 63 set), so the mantissa is top * 2^(p-64): the top two
+ * limbs, all others zero.  The value is |x| = top * 2^(E - BIAS - 64) with E the word's
+ * biased exponent.  bits holds the integer, or the IEEE bit pattern (binary32 in the low
+ * 32 bits); subnormal floats are exact too.  The reference forms the same words with
+ * frexp/ldexp and a count of leading zeros (fpa_warp_set.h:1-132). */
+FPA_HD uint32_t fpa_encode_scalar(int kind, uint64_t bits, uint64_t *top) {
+  uint32_t sign = 0;
+  uint64_t m;
+  int64_t s = 0;                               /* |x| = m * 2^s */
+  *top = 0;
+  if (kind == FPA_SRC_U64) {
+    m = bits;
+  } else if (kind == FPA_SRC_S64) {
+    sign = (uint32_t)(bits >> 63);
+    m = sign ? (uint64_t)0 - bits : bits;      /* INT64_MIN -> 2^63 */
+  } else if (kind == FPA_SRC_F32) {
+    const uint32_t f = (uint32_t)bits, e = (f >> 23) & 0xFFu;
+    sign = f >> 31;
+    m = f & 0x7FFFFFu;
+    if (e == 0xFFu) return m ? FPA_C_NAN : FPA_C_POS_INF - sign;
+    if (e) { m |= 1u << 23; s = (int64_t)e - 150; }
+    else s = -149;
+  } else {
+    const uint32_t e = (uint32_t)(bits >> 52) & 0x7FFu;
+    sign = (uint32_t)(bits >> 63);
+    m = bits & ((1ull << 52) - 1u);
+    if (e == 0x7FFu) return m ? FPA_C_NAN : FPA_C_POS_INF - sign;
+    if (e) { m |= 1ull << 52; s = (int64_t)e - 1075; }
+    else s = -1074;
+  }
+  if (m == 0) return FPA_C_ZERO;
+  const int lz = __builtin_clzll(m);
+  *top = m << lz;
+  return (uint32_t)(2 * (FPA_BIAS_LL + (64 - lz) + s) + sign);
 }
 
 #endif

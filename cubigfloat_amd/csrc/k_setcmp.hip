@@ -13,6 +13,11 @@
  *      the sticky bit, then the usual rounding with the add/sub/mul range rule.  The
  *      reference's set kernel is unfinished (fpa_warp_set.h:134-199); results equal
  *      mpfr_set (its gold, tests.c:580-605).  Special classes are copied.
+ * set_ui / set_si / set_float / set_double: one group per destination value; every lane
+ *      of the group reads the value's 4- or 8-byte source, fpa_encode_scalar gives the
+ *      exponent word and the left-aligned significand, the group's top lane writes it as
+ *      its two top limbs and every other limb is zero (reference fpa_warp_set.h:1-132:
+ *      one warp per value, the same words).
  */
 #include "fpa_kernel_common.h"
 
@@ -105,6 +110,25 @@ __global__ __launch_bounds__(256) void fpa_set_kernel(fpa_arguments a) {
   store_result(a, s, g.gl, re, H);
 }
 
+template <int K, int L>
+__global__ __launch_bounds__(256) void fpa_set_scalar_kernel(fpa_arguments a, int kind) {
+  static_assert(K >= 2, "the top lane holds the significand's two words");
+  Group<L> g;
+  const Slot s = group_slot<L>(g, a.count);
+  const uint64_t bits = kind == FPA_SRC_F32 ? (uint64_t)static_cast<const uint32_t *>(a.values)[s.v]
+                                            : static_cast<const uint64_t *>(a.values)[s.v];
+  uint64_t top;
+  const uint32_t re = fpa_encode_scalar(kind, bits, &top);
+  uint32_t x[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) x[i] = 0;
+  if (g.gl == L - 1) {
+    x[K - 1] = (uint32_t)(top >> 32);
+    x[K - 2] = (uint32_t)top;
+  }
+  store_result(a, s, g.gl, re, x);
+}
+
 }  // namespace fpa
 
 extern "C" int fpa_impl_launch_sgn(uint32_t count, const uint32_t *x_exps, int32_t *r, void *stream) {
@@ -114,15 +138,19 @@ extern "C" int fpa_impl_launch_sgn(uint32_t count, const uint32_t *x_exps, int32
   return (int)hipGetLastError();
 }
 
-/* cmp (op 0) or set (op 1) over args->count values with shape (K, L) of args->precision */
+/* over args->count values with shape (K, L) of args->precision: cmp (op 0), set (op 1), or
+ * set from the 4/8-byte sources at args->values (op 2 + FPA_SRC_*: ui, si, float, double) */
 extern "C" int fpa_impl_launch_setcmp(int op, const fpa_arguments *args, int K, int L, void *stream) {
   if (args->count == 0) return 0;
+  if (op < 0 || op > 2 + FPA_SRC_F64) return -1;
 #define FPA_SETCMP_CASE(n_, K_, L_)                                                               \
   if (args->precision == 32u * n_ && K == K_ && L == L_) {                                        \
     uint64_t waves = ((uint64_t)args->count + (64 / L_) - 1) / (64 / L_);                         \
     dim3 grid((uint32_t)((waves + 3) / 4));                                                       \
     if (op == 0) hipLaunchKernelGGL((fpa::fpa_cmp_kernel<K_, L_>), grid, dim3(256), 0, (hipStream_t)stream, *args); \
-    else hipLaunchKernelGGL((fpa::fpa_set_kernel<K_, L_>), grid, dim3(256), 0, (hipStream_t)stream, *args);         \
+    else if (op == 1) hipLaunchKernelGGL((fpa::fpa_set_kernel<K_, L_>), grid, dim3(256), 0, (hipStream_t)stream, *args); \
+    else hipLaunchKernelGGL((fpa::fpa_set_scalar_kernel<K_, L_>), grid, dim3(256), 0, (hipStream_t)stream, *args, \
+                            op - 2);                                                              \
     return (int)hipGetLastError();                                                                \
   }
   FPA_SHAPES(FPA_SETCMP_CASE)

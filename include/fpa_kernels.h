@@ -109,6 +109,10 @@ uint32_t fpa_scalar_round_inc(int mode, uint32_t sign, uint32_t lsb, uint32_t ro
 uint32_t fpa_scalar_recip_3by2(uint32_t d1, uint32_t d0);
 uint32_t fpa_scalar_div3by2(uint32_t u2, uint32_t u1, uint32_t u0, uint32_t d1, uint32_t d0, uint32_t v);
 uint32_t fpa_scalar_isqrt64(uint64_t u);
+/* Exponent word (or class code) and left-aligned significand of one fpa_set_* source:
+ * kind 0 uint64, 1 int64 (two's complement bits), 2 binary32 (bits in the low 32),
+ * 3 binary64 -- the encoding the set kernels write. */
+uint32_t fpa_scalar_encode(int kind, uint64_t bits, uint64_t *top);
 
 #ifdef __cplusplus
 }

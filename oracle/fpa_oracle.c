@@ -44,7 +44,13 @@
  * set (precision conversion) is unfinished in the reference (fpa_warp_set.h:134-199);
  * its gold is mpfr_set (tests.c:580-605): the source rounded once to the target
  * precision, special classes copied, and the add/sub/mul range rule on overflow.
+ * set_ui/si/float/double (fpa_warp_set.h:1-132): exact.  An integer's significand is the
+ * integer shifted left by its leading zeros, exponent word 2*(BIAS + bit length) + sign;
+ * a float or double goes through frexp to f in [1/2, 1) and ldexp(f, 64), as the
+ * reference kernels do (frexp, ldexp by 24 or 53), with NaN -> NAN, +-inf -> +-INF and
+ * zero of either sign -> ZERO.  Gold: mpfr_set_ui/si/flt/d (tests.c:487-578).
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -560,6 +566,41 @@ void orc_set_array(uint32_t count, uint32_t r_precision, uint32_t x_precision, i
     int64_t e = round_to(rl, nr, x_limbs + (size_t)i * nx, nx, scale, 0, sign, mode);
     r_exps[i] = pack_exp(e, sign, 5, 0x7FFFFFFFll);
     if (r_exps[i] <= C_SNAN) memset(rl, 0, 4 * (size_t)nr);
+  }
+}
+
+/* r = the fpa_set_* conversion of values (kind 0 uint64, 1 int64, 2 float, 3 double) */
+void orc_set_scalar_array(int kind, uint32_t count, uint32_t precision, const void *values, uint32_t *r_exps,
+                          uint32_t *r_limbs) {
+  int n = (int)(precision / 32);
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t *rl = r_limbs + (size_t)i * n;
+    memset(rl, 0, 4 * (size_t)n);
+    uint64_t mag;
+    int sign;
+    int64_t e;
+    if (kind <= 1) {
+      uint64_t u = ((const uint64_t *)values)[i];
+      sign = kind == 1 && (int64_t)u < 0;
+      mag = sign ? ~u + 1u : u;
+      if (mag == 0) { r_exps[i] = C_ZERO; continue; }
+      int bl = 64 - __builtin_clzll(mag);
+      e = bl;
+      mag <<= 64 - bl;
+    } else {
+      double x = kind == 2 ? (double)((const float *)values)[i] : ((const double *)values)[i];
+      if (isnan(x)) { r_exps[i] = C_NAN; continue; }
+      if (isinf(x)) { r_exps[i] = x < 0 ? C_NINF : C_PINF; continue; }
+      if (x == 0) { r_exps[i] = C_ZERO; continue; }
+      int ex;
+      double f = frexp(fabs(x), &ex);           /* |x| = f * 2^ex, f in [1/2, 1) */
+      mag = (uint64_t)ldexp(f, 64);             /* exact: f has at most 53 significant bits */
+      sign = x < 0;
+      e = ex;
+    }
+    rl[n - 1] = (uint32_t)(mag >> 32);
+    rl[n - 2] = (uint32_t)mag;
+    r_exps[i] = (uint32_t)(2 * (ORC_BIAS + e) + sign);
   }
 }
 

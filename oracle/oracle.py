@@ -166,7 +166,8 @@ def ref_lib():
     global _ref
     if _ref is None:
         _ref = ctypes.CDLL(REF_SO)
-        for name in ("add", "sub", "mul", "div", "sqrt", "neg", "sgn", "cmp"):
+        for name in ("add", "sub", "mul", "div", "sqrt", "neg", "sgn", "cmp",
+                     "set_ui", "set_si", "set_float", "set_double"):
             f = getattr(_ref, "tests_generate_" + name)
             f.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
             f.restype = ctypes.POINTER(_Tests)
@@ -197,6 +198,16 @@ def ref_generate(name, count, precision, source_precision=None):
 
     def arr(ptr, m):
         return np.ctypeslib.as_array(ptr, shape=(m,)).copy()
+
+    if name in SCALAR_KINDS:
+        # tests_add_mpfr_ui64/float/double (tests.c:225-283): values and gold only
+        ctype = {"set_ui": ctypes.c_uint64, "set_si": ctypes.c_int64, "set_float": ctypes.c_float,
+                 "set_double": ctypes.c_double}[name]
+        vals = np.ctypeslib.as_array(ctypes.cast(s.values, ctypes.POINTER(ctype)), shape=(count,)).copy()
+        out = {"values": vals, "gold_exps": arr(s.gold_exps, count), "gold_limbs": arr(s.gold_limbs, count * n)}
+        ref_lib().tests_free(t)
+        out["gold_limbs"].reshape(count, -1)[out["gold_exps"] == 2] = 0
+        return out
 
     out = {"a_exps": arr(s.a_exps, count), "a_limbs": arr(s.a_limbs, count * na),
            "gold_exps": arr(s.gold_exps, count)}
@@ -248,6 +259,22 @@ def mpfr_dot(precision, lengths, x_exps, x_limbs, y_exps, y_limbs, mode="rne", t
 
 
 # ----------------------------- set / sgn / cmp -----------------------------
+SCALAR_KINDS = {"set_ui": 0, "set_si": 1, "set_float": 2, "set_double": 3}
+SCALAR_DTYPES = {"set_ui": np.uint64, "set_si": np.int64, "set_float": np.float32, "set_double": np.float64}
+
+
+def set_scalar(name, precision, values):
+    """fpa_set_ui/si/float/double restated (exact); returns (r_exps, r_limbs)."""
+    values = np.ascontiguousarray(values, dtype=SCALAR_DTYPES[name])
+    count = len(values)
+    re = np.zeros(count, np.uint32)
+    rl = np.zeros(count * (precision // 32), np.uint32)
+    f = lib().orc_set_scalar_array
+    f.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, _u32p, _u32p]
+    f(SCALAR_KINDS[name], count, precision, values.ctypes.data, _p(re), _p(rl))
+    return re, rl
+
+
 def sgn(x_exps):
     x_exps = _c(x_exps)
     r = np.zeros(len(x_exps), np.int32)

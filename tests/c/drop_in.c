@@ -5,7 +5,8 @@
  * tests_compare_results exits(1) on the first mismatch, so the exit status is the verdict.
  *
  * usage: drop_in <op> <count> <precision> [source_precision]
- *        op = add|sub|mul|div|sqrt|sgn|cmp, or set (needs source_precision)
+ *        op = add|sub|mul|div|sqrt|neg|sgn|cmp|set_ui|set_si|set_float|set_double, or set
+ *        (needs source_precision)
  * sgn/cmp write into a device int32 array, as fpa_test.c does with cudaMalloc
  * (fpa_test.c:144-155); here hipMalloc.
  */
@@ -59,11 +60,37 @@ static int set(uint32_t count, uint32_t precision, uint32_t source) {
   return 0;
 }
 
+/* fpa_set_ui / si / float / double from the generator's host values (fpa_test.c:12-110) */
+static int set_scalar(const char *op, uint32_t count, uint32_t precision) {
+  tests_t *t;
+  if (!strcmp(op, "set_ui")) t = tests_generate_set_ui(count, precision);
+  else if (!strcmp(op, "set_si")) t = tests_generate_set_si(count, precision);
+  else if (!strcmp(op, "set_float")) t = tests_generate_set_float(count, precision);
+  else t = tests_generate_set_double(count, precision);
+  if (!t) return 3;
+  fpa_t r;
+  if (fpa_init(r, count, precision) != fpa_success) return 4;
+  fpa_result rc;
+  if (!strcmp(op, "set_ui")) rc = fpa_set_ui(r, (uint64_t *)t->values);
+  else if (!strcmp(op, "set_si")) rc = fpa_set_si(r, (int64_t *)t->values);
+  else if (!strcmp(op, "set_float")) rc = fpa_set_float(r, (float *)t->values);
+  else rc = fpa_set_double(r, (double *)t->values);
+  if (rc != fpa_success) { fprintf(stderr, "fpa_%s returned %d\n", op, rc); return 5; }
+  fpa_copy_array_to_host(r, t->r_exps, t->r_limbs);
+  tests_compare_results(t);
+  printf("%s tests passed: %u values at %u bits\n", op, count, precision);
+  free(t->values);
+  tests_free(t);
+  fpa_clear(r);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 4 && argc != 5) { fprintf(stderr, "usage: %s op count precision [source]\n", argv[0]); return 2; }
   const char *op = argv[1];
   uint32_t count = (uint32_t)atoi(argv[2]), precision = (uint32_t)atoi(argv[3]);
   if (!strcmp(op, "sgn") || !strcmp(op, "cmp")) return sgn_cmp(op, count, precision);
+  if (!strncmp(op, "set_", 4)) return set_scalar(op, count, precision);
   if (!strcmp(op, "set")) {
     if (argc != 5) { fprintf(stderr, "set needs a source precision\n"); return 2; }
     return set(count, precision, (uint32_t)atoi(argv[4]));
@@ -74,6 +101,7 @@ int main(int argc, char **argv) {
   else if (!strcmp(op, "mul")) t = tests_generate_mul(count, precision);
   else if (!strcmp(op, "div")) t = tests_generate_div(count, precision);
   else if (!strcmp(op, "sqrt")) t = tests_generate_sqrt(count, precision);
+  else if (!strcmp(op, "neg")) t = tests_generate_neg(count, precision);
   else { fprintf(stderr, "unknown op %s\n", op); return 2; }
   if (!t) return 3;
   fpa_t r, a, b;
@@ -81,6 +109,7 @@ int main(int argc, char **argv) {
   fpa_copy_array_to_device(a, t->a_exps, t->a_limbs);
   fpa_result rc;
   if (!strcmp(op, "sqrt")) rc = fpa_sqrt(r, a, fpa_round_ties_to_even);
+  else if (!strcmp(op, "neg")) rc = fpa_neg(r, a);
   else {
     fpa_init(b, count, precision);
     fpa_copy_array_to_device(b, t->b_exps, t->b_limbs);
@@ -96,6 +125,6 @@ int main(int argc, char **argv) {
   tests_free(t);
   fpa_clear(r);
   fpa_clear(a);
-  if (strcmp(op, "sqrt")) fpa_clear(b);
+  if (strcmp(op, "sqrt") && strcmp(op, "neg")) fpa_clear(b);
   return 0;
 }

@@ -8,6 +8,9 @@
 * setcmp/ref_set_<p>_<src>.npz, setcmp/ref_sgn_<p>.npz, setcmp/ref_cmp_<p>.npz -- the
   reference generator's set / sgn / cmp cases (tests.c:580-699) with its MPFR gold
   (mpfr_set in round to nearest even, mpfr_sgn, mpfr_cmp).
+* setcmp/ref_set_{ui,si,float,double}_<p>.npz, setcmp/ref_neg_<p>.npz -- the reference
+  generator's conversion and negation cases (tests.c:487-578, 700-725: source values, or
+  operands, with mpfr_set_ui/si/flt/d or mpfr_neg gold).
 Usage: python tests/golden/make_golden.py            (element-wise ops)
        python tests/golden/make_golden.py --setcmp   (set / sgn / cmp only)
 """
@@ -53,6 +56,10 @@ def main_setcmp():
         np.savez_compressed(os.path.join(out, f"ref_set_{p}_{src}.npz"), precision=np.uint32(p),
                             source_precision=np.uint32(src), **g)
     for name in ("sgn", "cmp"):
+        for p in (1024, 4096):
+            g = O.ref_generate(name, 64, p)
+            np.savez_compressed(os.path.join(out, f"ref_{name}_{p}.npz"), precision=np.uint32(p), **g)
+    for name in ("set_ui", "set_si", "set_float", "set_double", "neg"):
         for p in (1024, 4096):
             g = O.ref_generate(name, 64, p)
             np.savez_compressed(os.path.join(out, f"ref_{name}_{p}.npz"), precision=np.uint32(p), **g)

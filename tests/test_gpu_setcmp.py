@@ -133,7 +133,11 @@ def test_cmp_every_shape():
         B.close()
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(HERE, "golden", "setcmp", "*.npz"))),
+SETCMP_FIX = sorted(sum((glob.glob(os.path.join(HERE, "golden", "setcmp", pat))
+                         for pat in ("ref_set_[0-9]*.npz", "ref_sgn_*.npz", "ref_cmp_*.npz")), []))
+
+
+@pytest.mark.parametrize("path", SETCMP_FIX,
                          ids=lambda s: os.path.basename(s))
 def test_setcmp_fixtures(path):
     z = np.load(path)
@@ -159,3 +163,85 @@ def test_setcmp_fixtures(path):
         if B is not None:
             B.close()
     assert np.array_equal(got, np.sign(z["gold_exps"]))
+
+
+# ------------------------ fpa_set_ui / si / float / double, fpa_neg ------------------------
+import test_set_scalar as TS  # noqa: E402
+
+SCALAR_FIX = sorted(glob.glob(os.path.join(HERE, "golden", "setcmp", "ref_set_[usfd]*.npz")) +
+                    glob.glob(os.path.join(HERE, "golden", "setcmp", "ref_neg_*.npz")))
+
+
+def device_set_scalars(name, p, values):
+    R = F.FpaArray(len(values), p)
+    try:
+        assert F.set_scalars(R, name, values) == F.SUCCESS
+        return R.download()
+    finally:
+        R.close()
+
+
+@pytest.mark.parametrize("name", TS.KINDS)
+@pytest.mark.parametrize("p", F.PRECISIONS)
+def test_set_scalars_vs_oracle(name, p):
+    """Edge values (zeros, INT64_MIN, subnormals, NaN, infinities) and random bit patterns,
+    with the result limbs pre-filled with garbage so every limb the kernel writes is seen."""
+    vals = TS.edge_values(name, np.random.default_rng(p), extra=3000)
+    say(f"[gpu] {name} p={p} count={len(vals)}")
+    R = F.FpaArray(len(vals), p)
+    try:
+        rng = np.random.default_rng(1)
+        R.upload(rng.integers(9, 2 ** 32, len(vals), dtype=np.uint64).astype(np.uint32),
+                 rng.integers(0, 2 ** 32, len(vals) * (p // 32), dtype=np.uint64).astype(np.uint32))
+        assert F.set_scalars(R, name, vals) == F.SUCCESS
+        re_, rl = R.download()
+    finally:
+        R.close()
+    oe, ol = O.set_scalar(name, p, vals)
+    eq = O.results_equal(re_, rl, oe, ol, p)
+    assert eq.all(), f"{int((~eq).sum())} mismatches, first at {int(np.where(~eq)[0][0])}"
+    # class codes write zero limbs too
+    spec = re_ <= 8
+    assert not rl.reshape(-1, p // 32)[spec].any()
+
+
+@pytest.mark.parametrize("name", TS.KINDS)
+def test_set_scalars_every_shape(name):
+    """The conversions run on the neg kernel's shape for the precision: every such shape."""
+    for p in (1024, 4096, 8192):
+        vals = TS.edge_values(name, np.random.default_rng(7), extra=500)
+        keep = F.get_variant("neg", p)
+        try:
+            for v, (k, l) in enumerate(F.variants("neg", p)):
+                F.set_variant("neg", p, v)
+                say(f"[gpu] {name} p={p} shape {k}x{l}")
+                re_, rl = device_set_scalars(name, p, vals)
+                oe, ol = O.set_scalar(name, p, vals)
+                assert O.results_equal(re_, rl, oe, ol, p).all(), (p, k, l)
+        finally:
+            F.set_variant("neg", p, keep)
+
+
+def test_set_scalars_count_zero_and_one():
+    for name in TS.KINDS:
+        R = F.FpaArray(0, 1024)
+        assert F.set_scalars(R, name, np.zeros(0)) == F.SUCCESS
+        R.close()
+        vals = TS.edge_values(name, np.random.default_rng(3), extra=0)[3:4]
+        re_, rl = device_set_scalars(name, 1024, vals)
+        oe, ol = O.set_scalar(name, 1024, vals)
+        assert O.results_equal(re_, rl, oe, ol, 1024).all()
+
+
+@pytest.mark.parametrize("path", SCALAR_FIX, ids=lambda s: os.path.basename(s))
+def test_set_scalar_and_neg_fixtures(path):
+    """The reference generator's set_ui/si/float/double and neg cases with its MPFR gold."""
+    z = np.load(path)
+    p = int(z["precision"])
+    name = os.path.basename(path)[4:].rsplit("_", 1)[0]
+    say(f"[gpu] fixture {os.path.basename(path)}")
+    if name == "neg":
+        re_, rl = F.run_op("neg", p, z["a_exps"], z["a_limbs"])
+    else:
+        re_, rl = device_set_scalars(name, p, z["values"])
+    assert O.results_equal(re_, rl, z["gold_exps"], z["gold_limbs"], p).all()

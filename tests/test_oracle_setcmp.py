@@ -175,7 +175,8 @@ def test_tables_follow_reference_text():
 FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setcmp")
 
 
-@pytest.mark.parametrize("path", sorted(__import__("glob").glob(os.path.join(FIX, "*.npz"))),
+@pytest.mark.parametrize("path", sorted(sum((__import__("glob").glob(os.path.join(FIX, pat))
+                                              for pat in ("ref_set_[0-9]*.npz", "ref_sgn_*.npz", "ref_cmp_*.npz")), [])),
                          ids=lambda s: os.path.basename(s))
 def test_oracle_matches_setcmp_fixtures(path):
     z = np.load(path)
