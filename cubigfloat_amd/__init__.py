@@ -201,6 +201,19 @@ def set_scalars(r, name, values):
     return fn(r.ref, v.ctypes.data_as(ctypes.POINTER(ct)))
 
 
+lib.fpa_set_scalars_noalloc.argtypes = [_P, ctypes.c_int, ctypes.c_void_p]
+NOALLOC_MAX_VALUES = 1 << 19
+
+
+def set_scalars_noalloc(r, name, values):
+    """fpa_set_scalars_noalloc: the fpa_set_* conversion `name` with no device allocation."""
+    _, dt, _ = SCALAR_SETTERS[name]
+    v = np.ascontiguousarray(values, dtype=dt)
+    assert v.size == r.count
+    kind = ("set_ui", "set_si", "set_float", "set_double").index(name)
+    return lib.fpa_set_scalars_noalloc(r.ref, kind, v.ctypes.data)
+
+
 def scalar_encode(kind, bits):
     """(exponent word, left-aligned significand) of one fpa_set_* source (host helper)."""
     top = ctypes.c_uint64(0)

@@ -19,6 +19,11 @@
  *      its two top limbs and every other limb is zero (reference fpa_warp_set.h:1-132:
  *      one warp per value, the same words).
  */
+#include <mutex>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
 #include "fpa_kernel_common.h"
 
 namespace fpa {
@@ -156,4 +161,81 @@ extern "C" int fpa_impl_launch_setcmp(int op, const fpa_arguments *args, int K, 
   FPA_SHAPES(FPA_SETCMP_CASE)
 #undef FPA_SETCMP_CASE
   return -1;
+}
+
+/* ------------------ fpa_set_* from host values without allocating ------------------ */
+
+/* Staging for fpa_set_scalars_noalloc: the 4- or 8-byte sources of one call.  A file-scope
+ * device array: each GPU the library runs on holds its own copy (8 B x
+ * FPA_NOALLOC_MAX_VALUES = 4 MiB) until the process ends.  Host code never passes the
+ * array's name to a kernel or a copy (on the host it is only a stand-in); it uses the
+ * device address that hipGetSymbolAddress returns for the current device. */
+static __device__ __align__(256) uint64_t scratch[FPA_NOALLOC_MAX_VALUES];
+
+#define FPA_NOALLOC_MAX_DEVICES 64
+static void *scratch_ptr[FPA_NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+/* device address of `scratch` on the current device, looked up once per device */
+static void *scratch_on_device() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= FPA_NOALLOC_MAX_DEVICES) {
+    fprintf(stderr, "\nfpa_set_scalars_noalloc: no usable current device (%d, %s)\n", dev, hipGetErrorString(e));
+    exit(1);
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void *p = NULL;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "\nfpa_set_scalars_noalloc: hipGetSymbolAddress failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      exit(1);
+    }
+    scratch_ptr[dev] = p;
+  }
+  return scratch_ptr[dev];
+}
+
+extern "C" fpa_result fpa_set_scalars_noalloc(fpa_t r, int kind, const void *values) {
+  if (kind < FPA_SRC_U64 || kind > FPA_SRC_F64) return fpa_math_error;
+  if (!fpa_precision_supported(r->precision)) return fpa_invalid_precisions;
+  if (r->count > FPA_NOALLOC_MAX_VALUES) {
+    fprintf(stderr, "fpa_set_scalars_noalloc: %u values exceed the staging limit of %u values\n", r->count,
+            (unsigned)FPA_NOALLOC_MAX_VALUES);
+    return fpa_invalid_counts;
+  }
+  if (r->count == 0) return fpa_success;
+  void *ptr = scratch_on_device();
+  hipStream_t st = (hipStream_t)fpa_get_stream();
+  const size_t bytes = (kind == FPA_SRC_F32 ? 4u : 8u) * (size_t)r->count;
+  /* every staged word the kernel reads is written here first: nothing needs clearing */
+  hipError_t e = hipMemcpyAsync(ptr, values, bytes, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    fprintf(stderr, "\nfpa_set_scalars_noalloc: staging copy failed: %s\n", hipGetErrorString(e));
+    exit(1);
+  }
+  fpa_arguments a;
+  memset(&a, 0, sizeof a);
+  a.count = r->count;
+  a.precision = r->precision;
+  a.r_exps = r->gpu_exps;
+  a.r_limbs = r->gpu_limbs;
+  a.values = ptr;
+  int K = 0, L = 0;
+  fpa_kernel_variant_shape(FPA_OP_NEG, r->precision,
+                           fpa_kernel_get_variant_for_count(FPA_OP_NEG, r->precision, r->count), &K, &L);
+  const int rc = fpa_impl_launch_setcmp(2 + kind, &a, K, L, st);
+  if (rc != 0) {
+    fprintf(stderr, "\nfpa_set_scalars_noalloc: launch failed (%d)\n", rc);
+    exit(1);
+  }
+  /* the next call on this device overwrites the staging words: wait for the kernel */
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    fprintf(stderr, "\nfpa_set_scalars_noalloc: %s\n", hipGetErrorString(e));
+    exit(1);
+  }
+  return fpa_success;
 }

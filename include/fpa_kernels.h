@@ -92,6 +92,23 @@ int fpa_kernel_tuning_applied(void);
  * stores the mean kernel time in ms in *ms.  Returns 0, or the launch / HIP error. */
 int fpa_kernel_time(int op, const fpa_arguments *args, int variant, int reps, void *stream, float *ms);
 
+/* Largest count fpa_set_scalars_noalloc accepts. */
+#define FPA_NOALLOC_MAX_VALUES (1u << 19)
+
+/* fpa_set_ui / fpa_set_si / fpa_set_float / fpa_set_double (kind 0..3, as
+ * fpa_scalar_encode) without allocating: no counterpart in the reference, whose set
+ * calls cudaMalloc and cudaFree a staging buffer each time (fpa_dispatch.h:8-86).  For
+ * callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops).  The 4- or 8-byte sources are staged in a fixed device buffer of
+ * FPA_NOALLOC_MAX_VALUES values that each GPU holds from its first call until the
+ * process ends; the call copies them there on the fpa stream, launches the same kernel
+ * as fpa_set_*, and waits for it.  Results are identical to fpa_set_*.
+ * Returns fpa_invalid_counts (with a message) above FPA_NOALLOC_MAX_VALUES values,
+ * fpa_invalid_precisions for an unsupported precision, fpa_math_error for a bad kind.
+ * All calls on one GPU share the staging buffer: calls that overlap on one GPU (from
+ * several host threads) race and must be serialised by the caller. */
+fpa_result fpa_set_scalars_noalloc(fpa_t r, int kind, const void *values);
+
 /* Stream used by the fpa.h functions (a hipStream_t; NULL = null stream). */
 void fpa_set_stream(void *stream);
 void *fpa_get_stream(void);
