@@ -5,8 +5,9 @@ Workload (default): 2^20 operand pairs at 4096-bit precision (128 limbs), expone
 uniform in [-1024, 1024], 1% exact zeros, 5% near-cancellation pairs; one step = one
 fpa_mul launch over all 2^20 pairs, operands resident in HBM.  The same run times the
 other plain per-op launches of that workload (add/sub/div/sqrt, 1024-bit add/sub/mul),
-checks a sample of every op bit-exact against MPFR, and times MPFR on the host cores
-as the CPU baseline.
+checks every result of every timed launch bit-exact against MPFR gold computed on all
+host threads (full_launch_checks_vs_mpfr, mismatches_vs_mpfr), and times MPFR on the host
+cores as the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--count N] [--no-cpu]
 Prints one JSON line.
@@ -100,6 +101,7 @@ def time_op(F, hip, op, p, count, steps, warmup, kind, seed, check_sample):
             x.close()
     n = p // 32
     sample = slice(0, check_sample)
+    full = ((ae, al, be, bl), (re, rl))
     res = {"op": op, "precision": p, "count": count, "ms_per_launch": float(np.mean(per)),
            "ms_min": float(np.min(per)), "wall_s": wall, "upload_s": t_up}
     res["ops_per_s"] = count / (res["ms_per_launch"] / 1e3)
@@ -113,7 +115,19 @@ def time_op(F, hip, op, p, count, steps, warmup, kind, seed, check_sample):
         res["frac_of_hbm_peak"] = res["gb_per_s"] / HBM_GBS
     sample_in = (ae[sample], al[sample.start * n:sample.stop * n],
                  None if be is None else be[sample], None if bl is None else bl[sample.start * n:sample.stop * n])
-    return res, sample_in, (re[sample], rl[sample.start * n:sample.stop * n])
+    return res, sample_in, (re[sample], rl[sample.start * n:sample.stop * n]), full
+
+
+def full_check(op, p, full, threads):
+    """Every result of the timed launch against MPFR gold (all host threads): returns
+    {"checked": elements MPFR could hold, "mismatches": n, "seconds": t}."""
+    from oracle import oracle as O
+    (ae, al, be, bl), (re, rl) = full
+    t0 = time.perf_counter()
+    ge, gl, ok = O.mpfr_gold(op, p, ae, al, be, bl, threads=threads)
+    eq = O.results_equal(re, rl, ge, gl, p)
+    return {"checked": int(ok.sum()), "of": int(len(ok)), "mismatches": int((~eq & ok).sum()),
+            "seconds": time.perf_counter() - t0}
 
 
 def cpu_baseline(op, p, sample_in, gpu_out, threads):
@@ -165,7 +179,7 @@ def kernels_only(F, hip, args):
     ops = [o for o in args.only.split(",") if o] or ["neg", "mul", "add"]
     for op in ops:
         print(f"[bench] kernels-only {op} 4096 x{args.count}", file=sys.stderr, flush=True)
-        r, _, _ = time_op(F, hip, op, 4096, args.count, args.steps, 1, 2, 2, 16)
+        r, _, _, _ = time_op(F, hip, op, 4096, args.count, args.steps, 1, 2, 2, 16)
         r["algorithmic_bytes_per_launch"] = algorithmic(op, 4096)[1] * args.count
         res.append(r)
     print(json.dumps({"kernels_only": res}), flush=True)
@@ -205,7 +219,7 @@ def bench_config3(F, hip, nops, steps, cpu, threads):
     res["batch_equals_plain"] = same
     if cpu:
         from oracle import oracle as O
-        t_cpu, sample_ok = 0.0, True
+        t_cpu, sample_ok, bad = 0.0, True, 0
         O.cpu_set_threads(threads)
         inv = {v: k for k, v in F.OPS.items()}
         for op, p, m, idx, e1, l1, pos in W.split_by_op(ops, precs, modes, ae, al):
@@ -221,12 +235,15 @@ def bench_config3(F, hip, nops, steps, cpu, threads):
             O.cpu_op(inv[op], R, A, Bc, ["rz", "ru", "rd", "rne", "rna"][m])
             t_cpu += time.perf_counter() - t0
             ce, cl = R.store(inv[op])
-            sample_ok &= bool(O.results_equal(be_[idx], bl_[pos], ce, cl, p).all())
+            eq = O.results_equal(be_[idx], bl_[pos], ce, cl, p)
+            sample_ok &= bool(eq.all())
+            bad += int((~eq).sum())
             for x in (A, Bc, R):
                 if x is not None:
                     x.close()
         res["cpu_mpfr"] = {"ops_per_s": nops / t_cpu, "cores": threads}
         res["bit_exact_vs_mpfr"] = sample_ok
+        res["mismatches_vs_mpfr"] = bad
     B.close()
     return res
 
@@ -254,15 +271,17 @@ def bench_config4(F, hip, nchains, steps, cpu, threads):
            "ms": ms, "dots_per_s": nchains / (ms / 1e3), "mul_add_pairs_per_s": total / (ms / 1e3)}
     if cpu:
         from oracle import oracle as O
-        k = 512
+        k = nchains
         off = np.concatenate([[0], np.cumsum(lengths[:k], dtype=np.int64)])
         n = 64
         t0 = time.perf_counter()
         me, ml = O.mpfr_dot(2048, lengths[:k], xe[:off[-1]], xl[:off[-1] * n], ye[:off[-1]], yl[:off[-1] * n],
                             threads=threads)
         dt = time.perf_counter() - t0
-        res["cpu_mpfr"] = {"dots_per_s": k / dt, "cores": threads, "sample": f"first {k} chains"}
-        res["bit_exact_vs_mpfr"] = bool(O.results_equal(re[:k], rl[:k * n], me, ml, 2048).all())
+        res["cpu_mpfr"] = {"dots_per_s": k / dt, "cores": threads, "sample": f"all {k} chains"}
+        eq = O.results_equal(re[:k], rl[:k * n], me, ml, 2048)
+        res["bit_exact_vs_mpfr"] = bool(eq.all())
+        res["mismatches_vs_mpfr"] = int((~eq).sum())
     for a in (X, Y, R):
         a.close()
     plan.close()
@@ -347,7 +366,11 @@ def main():
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     p, count = 4096, args.count
     print(f"[bench] mul {p}-bit x{count}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    head, sample_in, gpu_out = time_op(F, hip, "mul", p, count, args.steps, args.warmup, 2, 2, 4096)
+    head, sample_in, gpu_out, full = time_op(F, hip, "mul", p, count, args.steps, args.warmup, 2, 2, 4096)
+    full_checks = {}
+    if not args.no_cpu:
+        full_checks["mul4096"] = full_check("mul", p, full, threads)
+    del full
 
     extras = []
     others = [("add", 4096, 2), ("sub", 4096, 2), ("div", 4096, 2), ("sqrt", 4096, 2),
@@ -359,7 +382,10 @@ def main():
             continue
         cnt = count if kind == 2 else 65536
         print(f"[bench] {op} {pp}-bit x{cnt}", file=sys.stderr, flush=True)
-        r, si, go = time_op(F, hip, op, pp, cnt, max(3, args.steps // 4), 1, kind, kind, 2048)
+        r, si, go, full = time_op(F, hip, op, pp, cnt, max(3, args.steps // 4), 1, kind, kind, 2048)
+        if not args.no_cpu:
+            full_checks[f"{op}{pp}"] = full_check(op, pp, full, threads)
+        del full
         if not args.no_cpu:
             rate, _, ok, bad = cpu_baseline(op, pp, si, go, threads)
             r["cpu_mpfr_ops_per_s"] = rate
@@ -417,6 +443,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "checks_bit_exact_vs_mpfr": checks,
+        "full_launch_checks_vs_mpfr": full_checks,
+        "mismatches_vs_mpfr": sum(c["mismatches"] for c in full_checks.values()) if full_checks else None,
         "extras": extras,
         "configs": configs,
         "wall_s": head["wall_s"],
