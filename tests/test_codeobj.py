@@ -7,6 +7,14 @@ reached through flat loads/stores) are the only ones that ever failed on the dev
 so the build keeps them out: the dot-chain kernel is instantiated only for shapes whose
 body inlines, and this test fails if any call appears.
 
+No kernel may contain a FLAT (generic-address) load or store either: every LDS access
+must be a DS instruction and every HBM access a global one.  A FLAT instruction picks
+its aperture (global, LDS, scratch) from the high bits of its base address, without the
+instruction's immediate offset; a guarded read of lds[w+1] at w = -1, compiled as
+flat_load (lds - 4) offset:4, then leaves the LDS aperture when lds is LDS address 0 and
+faults.  Generic pointers appear only when a core is compiled as a real call, which is
+how the two kernels that faulted were built.
+
 The element-wise kernels of one op may be split over several compilation units (parts
 that build in parallel): every (op, shape) the dispatcher can launch must be compiled
 exactly once, in one code object."""
@@ -43,8 +51,8 @@ def code_objects(tmp_path):
 
 @pytest.mark.skipif(not (os.path.exists(LIB) and shutil.which("objcopy") and os.path.exists(f"{LLVM}/llvm-objdump")),
                     reason="library or LLVM tools missing")
-def test_no_kernel_calls_a_function(tmp_path):
-    calls = {}
+def test_no_kernel_calls_a_function_or_uses_flat_addressing(tmp_path):
+    calls, flat = {}, {}
     kernels = 0
     for co in code_objects(tmp_path):
         dis = subprocess.run([f"{LLVM}/llvm-objdump", "-d", str(co)], check=True, capture_output=True,
@@ -58,8 +66,11 @@ def test_no_kernel_calls_a_function(tmp_path):
                 continue
             if "s_swappc_b64" in line:
                 calls[cur] = calls.get(cur, 0) + 1
+            if re.search(r"\bflat_(load|store|atomic)", line):
+                flat[cur] = flat.get(cur, 0) + 1
     assert kernels > 0
     assert not calls, f"kernels with device-function calls: {sorted(calls)[:8]}"
+    assert not flat, f"kernels with FLAT memory instructions: {sorted(flat)[:8]}"
 
 
 def _shapes():
