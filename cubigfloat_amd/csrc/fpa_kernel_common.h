@@ -32,6 +32,7 @@ struct ScratchWords {
   static constexpr int raw = OP == 0 ? 0
                            : OP <= 2 ? n + 1
                            : OP == 3 ? mul_lds_words(K, L)
+                           : OP == 4 ? div_lds_words(K, L)
                            : n + 1;
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
   static constexpr int value = ((raw + 3) / 4) * 4 + 4;

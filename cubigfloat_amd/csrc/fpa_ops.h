@@ -879,12 +879,288 @@ __device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea
   chunk_select(Qr, true, M, M);
 }
 
+/* ------------------- div with lazy radix-2^27 column sums ------------------- */
+/*
+ * Long division in radix beta = 2^27 with the remainder kept as signed 64-bit column sums
+ * that are never carry-propagated inside the digit loop (the same idea as mul's column
+ * sums).  Divisor and remainder are recoded to KD = ceil(32K/27) digits per lane, aligned
+ * to the top (the divisor's top digit has its top bit set).  Each step:
+ *   1. R <- R * beta: every column moves up one position (a register renaming inside the
+ *      unrolled trip; one 64-bit column crosses each lane boundary); the column leaving
+ *      the top is folded into the new top column (c_top * beta + c, exact modulo 2^64,
+ *      and small in truth because the remainder stays below 2D);
+ *   2. the group's top lane estimates the digit q = floor(V / Dt) in FP64 from its top two
+ *      columns V = c1 * beta + c0 and the divisor's top two digits Dt (the neglected
+ *      columns below contribute less than 2^-17 of a digit), and broadcasts it;
+ *   3. every lane subtracts q * d from its columns: one v_mad_i64_i32 per column with the
+ *      negated divisor digits.
+ * The estimate can be one too large or too small, so the digits are redundant (signed,
+ * |q| < 2 beta) and the remainder stays in (-D, 2D); |q * d| < 2^55, so a column that lives
+ * ND <= 255 steps stays below 2^63, and longer values renormalise every 128 steps by one
+ * carry level.  At the end, the FP64 estimate of R/D decides (R in (0, D): no correction,
+ * remainder nonzero) unless it lies within 2^-16 of 0 or 1; then R is normalised exactly
+ * and corrected by one D (quotient -1 or +1).  The quotient digits (one per step, in LDS)
+ * are carry-normalised only when some digit lies outside [0, beta).  The reference's
+ * division (fpa_warp_div.h:127-201) runs 32-bit digits with exact remainders instead.
+ */
+constexpr int DIV_RADIX = 27;
+
+__host__ __device__ constexpr int div_digits_per_lane(int K) { return (32 * K + DIV_RADIX - 1) / DIV_RADIX; }
+/* quotient digits: enough for the p-bit mantissa, the 32-bit round word and a bit more */
+__host__ __device__ constexpr int div_steps(int n) { return (32 * (n + 1) + DIV_RADIX - 1) / DIV_RADIX; }
+/* zero words staged below a value so that the top-aligned digits' padding reads zeros */
+__host__ __device__ constexpr int div_pad_words(int K, int L) {
+  return (DIV_RADIX * div_digits_per_lane(K) * L - 32 * K * L + 31) / 32;
+}
+
+/* LDS words per value: recoding stage or the quotient digits (+ topbit, + two zeros) */
+__host__ __device__ constexpr int div_lds_words(int K, int L) {
+  return div_pad_words(K, L) + K * L + 2 > div_steps(K * L) + 3 ? div_pad_words(K, L) + K * L + 2
+                                                                   : div_steps(K * L) + 3;
+}
+
+/* x as held by group lane src, 64 bits */
+template <int L>
+__device__ __forceinline__ int64_t bcast64(const Group<L> &g, int64_t x, int src) {
+  const uint32_t lo = g.bcast((uint32_t)x, src), hi = g.bcast((uint32_t)((uint64_t)x >> 32), src);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* signed 64-bit value as a double (3 VALU: two conversions and one fma) */
+__device__ __forceinline__ double i64_to_f64(int64_t x) {
+  return __builtin_fma((double)(int32_t)((uint64_t)x >> 32), 4294967296.0, (double)(uint32_t)x);
+}
+
+/* Digits of X (n = K*L words) in radix 2^27 aligned to the top of KD*L digits: digit k of
+ * the value = bits [27k - PAD, 27k - PAD + 27) of X, PAD = 27*KD*L - 32n (bits below X's bit
+ * 0 read as zero).  lds: ZW + n + 2 words, ZW = ceil(PAD / 32). */
+template <int K, int L, int KD>
+__device__ __forceinline__ void recode_digits_top(const Group<L> &g, uint32_t *lds, const uint32_t (&X)[K],
+                                                  uint32_t (&D)[KD]) {
+  constexpr int n = K * L, PAD = DIV_RADIX * KD * L - 32 * n, ZW = (PAD + 31) / 32;
+  constexpr uint32_t M = (1u << DIV_RADIX) - 1;
+  static_assert(PAD >= 0, "digits cover the value");
+  lds_store(lds + ZW + g.gl * K, X);
+  if (g.gl == 0) {
+#pragma unroll
+    for (int i = 0; i < ZW; i++) lds[i] = 0;
+  }
+  if (g.gl == L - 1) { lds[ZW + n] = 0; lds[ZW + n + 1] = 0; }
+  wsync();
+  /* the lane's digits are bits [b, b + 27*KD) of the staged string (its bit 32*ZW is X's
+   * bit 0); the highest word read is floor(b/32) + NW <= ZW + n + 1 */
+  constexpr int NW = (DIV_RADIX * KD + 31) / 32;
+  const int b = DIV_RADIX * KD * g.gl - PAD + 32 * ZW;
+  const uint32_t *src = lds + (b >> 5);
+  const uint32_t sh = (uint32_t)b & 31u;
+  uint32_t win[NW + 1];
+  uint32_t lo = src[0];
+#pragma unroll
+  for (int i = 0; i < NW; i++) {
+    const uint32_t hi = src[i + 1];
+    win[i] = __builtin_amdgcn_alignbit(hi, lo, sh);
+    lo = hi;
+  }
+  win[NW] = 0;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    const int j = (DIV_RADIX * k) >> 5, s2 = (DIV_RADIX * k) & 31;
+    D[k] = (s2 + DIV_RADIX <= 32 ? win[j] >> s2 : __builtin_amdgcn_alignbit(win[j + 1], win[j], (uint32_t)s2)) & M;
+  }
+  wsync();
+}
+
+/* Exact normalisation, in place, of N signed columns per lane (lane gl holds positions
+ * gl*N .. +N-1) into digits in [0, 2^27); returns the carry out of the top lane (the value's
+ * top part): value = sum C_k beta^k + top * beta^(N*L).  Columns below 2^63 in magnitude. */
+template <int N, int L>
+__device__ __forceinline__ int64_t normalize_exact(const Group<L> &g, int64_t (&C)[N]) {
+  constexpr int64_t M = ((int64_t)1 << DIV_RADIX) - 1;
+  int64_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    const int64_t v = C[i] + carry;
+    C[i] = v & M;
+    carry = v >> DIV_RADIX;
+  }
+  /* lane t's carry out moves into lane t+1, one lane at a time (the other lanes add zero
+   * to digits that are already normalised, which changes nothing) */
+  for (int t = 0; t < L - 1; t++) {
+    const int64_t c = bcast64(g, carry, t);
+    int64_t cin = g.gl == t + 1 ? c : 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const int64_t v = C[i] + cin;
+      C[i] = v & M;
+      cin = v >> DIV_RADIX;
+    }
+    carry += cin;
+  }
+  return bcast64(g, carry, L - 1);
+}
+
+/* (re, Qr) = a / b; lds: div_lds_words(K, L) words */
+template <int K, int L>
+__device__ void core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+                              uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
+  constexpr int n = K * L, p = 32 * n;
+  constexpr int KD = div_digits_per_lane(K), ND = KD * L, S = div_steps(n);
+  constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
+  constexpr int64_t BETA = (int64_t)1 << DIV_RADIX;
+  static_assert(KD >= 2, "the top lane holds the two top columns");
+  static_assert(27 * S >= p + 32, "quotient digits cover the mantissa and the round word");
+  const uint32_t act = k_div_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  if (act != FPA_A_DIV) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) Qr[i] = 0;
+    return;
+  }
+  const uint32_t sign = (ea ^ eb) & 1u;
+  int64_t E = (int64_t)(ea >> 1) - (int64_t)(eb >> 1) + FPA_BIAS_LL;
+  /* leading quotient bit: A >= B; the remainder R0 = A or A - B is below B */
+  uint32_t R0[K];
+  const uint32_t below = g.sub(R0, A, B);
+  const uint32_t topbit = below ? 0u : 1u;
+  if (below) {
+#pragma unroll
+    for (int i = 0; i < K; i++) R0[i] = A[i];
+  }
+  uint32_t dd[KD], rd[KD];
+  recode_digits_top<K, L, KD>(g, lds, B, dd);
+  recode_digits_top<K, L, KD>(g, lds, R0, rd);
+  int32_t nd[KD];
+  int64_t C[KD];
+#pragma unroll
+  for (int i = 0; i < KD; i++) { nd[i] = -(int32_t)dd[i]; C[i] = (int64_t)rd[i]; }
+  /* divisor's top two digits (meaningful in the top lane) */
+  const double inv = 1.0 / __builtin_fma((double)dd[KD - 1], (double)BETA, (double)dd[KD - 2]);
+  const bool top = g.gl == L - 1;
+  int32_t *qd = reinterpret_cast<int32_t *>(lds);   /* quotient digit of step j at S-1-j */
+
+  auto step = [&](int j) __attribute__((always_inline)) {
+    /* R <- R * beta: columns move up; the column leaving a lane enters the next lane's
+     * bottom; the one leaving the top lane folds into the new top column */
+    const int64_t so = C[KD - 1];
+    const uint32_t inlo = g.from_prev_row((uint32_t)so), inhi = g.from_prev_row((uint32_t)((uint64_t)so >> 32));
+#pragma unroll
+    for (int i = KD - 1; i > 0; i--) C[i] = C[i - 1];
+    C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
+    const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
+    C[KD - 1] = (int64_t)((uint64_t)C[KD - 1] + fold);
+    /* digit estimate from the top two columns (top lane), broadcast */
+    const double V = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2]));
+    int32_t q = (int32_t)__builtin_floor(V * inv);
+    q = (int32_t)g.bcast_top((uint32_t)q);
+#pragma unroll
+    for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nd[i] + C[i];
+    qd[S - 1 - j] = q;
+    /* keep the scheduler from running the digit chain of later steps ahead of this step's
+     * multiply-adds: that lookahead holds several steps' columns live at once and costs
+     * occupancy, while other waves hide the chain's latency anyway */
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (ND > 255) {
+      /* one carry level every 128 steps keeps the columns far below 2^63 */
+      if ((j & 127) == 127) {
+        int64_t car[KD];
+#pragma unroll
+        for (int i = 0; i < KD; i++) car[i] = C[i] >> DIV_RADIX;
+        const uint32_t clo = g.from_prev_row((uint32_t)car[KD - 1]);
+        const uint32_t chi = g.from_prev_row((uint32_t)((uint64_t)car[KD - 1] >> 32));
+        const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
+#pragma unroll
+        for (int i = KD - 1; i >= 0; i--) {
+          const int64_t keep = (top && i == KD - 1) ? C[i] : (C[i] & (int64_t)MASK);
+          C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
+        }
+      }
+    }
+  };
+  wsync();   /* the recoding reads of lds are done before the digits overwrite them */
+  int j0 = 0;
+  for (; j0 + KD <= S; j0 += KD) repeat<KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+  repeat<S % KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+
+  /* final remainder R = sum C_k beta^k lies in (-D, 2D); its estimate t ~ R/D */
+  const double t = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2])) * inv;
+  const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
+  uint32_t rem_nz = 1;
+  int32_t delta = 0;
+  if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
+    /* exact: normalise R; add or subtract D once; the quotient moves by -1 / +1 */
+    const int64_t T = normalize_exact<KD, L>(g, C);
+    int64_t sgn = 0;                       /* +1: subtract D, -1: add D */
+    if (T < 0) {
+      sgn = -1;
+    } else if (T > 0) {
+      sgn = 1;
+    } else {
+      /* compare R with D: the highest lane with a differing digit decides */
+      int c = 0;
+#pragma unroll
+      for (int i = KD - 1; i >= 0; i--)
+        if (c == 0 && C[i] != (int64_t)-nd[i]) c = C[i] > (int64_t)-nd[i] ? 1 : -1;
+      const uint64_t diff = g.ballot(c != 0);
+      const int cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
+      sgn = cmp >= 0 ? 1 : 0;
+    }
+    delta = (int32_t)sgn;
+#pragma unroll
+    for (int i = 0; i < KD; i++) C[i] += sgn * (int64_t)nd[i];
+    normalize_exact<KD, L>(g, C);
+    uint32_t nz = 0;
+#pragma unroll
+    for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
+    rem_nz = g.any(nz != 0) ? 1u : 0u;
+  }
+  wsync();
+  if (delta != 0 && g.gl == 0) qd[0] += delta;
+  if (g.gl == 0) { qd[S] = (int32_t)topbit; qd[S + 1] = 0; qd[S + 2] = 0; }
+  wsync();
+  /* quotient digits outside [0, beta) (a corrected estimate): carry-normalise them */
+  {
+    bool bad = false;
+    for (int k = g.gl; k < S; k += L) bad |= (uint32_t)qd[k] > MASK;
+    if (__builtin_expect(__ballot(bad) != 0, 0)) {
+      constexpr int CS = (S + L - 1) / L;
+      int64_t X[CS];
+#pragma unroll
+      for (int i = 0; i < CS; i++) {
+        const int k = g.gl * CS + i;
+        X[i] = k < S ? (int64_t)qd[k] : 0;
+      }
+      normalize_exact<CS, L>(g, X);   /* the quotient is below beta^S: no carry out */
+      wsync();
+#pragma unroll
+      for (int i = 0; i < CS; i++) {
+        const int k = g.gl * CS + i;
+        if (k < S) qd[k] = (int32_t)X[i];
+      }
+      wsync();
+    }
+  }
+  /* F = topbit * beta^S + Q; the mantissa is its top p bits from the leading one */
+  const uint32_t *qu = reinterpret_cast<const uint32_t *>(qd);
+  const int base = topbit ? 27 * S - p + 1 : 27 * S - p;
+  uint32_t Mw[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) Mw[i] = digits_bits(qu, base + 32 * (g.gl * K + i));
+  const uint32_t rw = digits_bits(qu, base - 32);
+  const int lim = base - 32;   /* sticky bits [0, lim) lie in digit 0: lim <= 27 */
+  const uint32_t sticky = rem_nz | ((lim > 0 && (qu[0] & ((1u << lim) - 1u)) != 0) ? 1u : 0u);
+  wsync();
+  if (topbit) E += 1;
+  if (round_mantissa<K, L>(g, Mw, mode, sign, rw, sticky)) E += 1;
+  re = pack_exp(E, sign, 10, 0x7FFFFFFEll);
+  chunk_select(Qr, true, Mw, Mw);
+}
+
 template <int K, int L>
 __device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
-  core_div<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  core_div_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
   store_result(a, s, g.gl, re, R);
 }
 
