@@ -11,10 +11,9 @@
  *            rounds broadcast a's digit chunk r, windows of column sums shift down one
  *            lane per round (group lane 0's low window is complete), carries are
  *            normalised once at the end.
- *   div      fpa_warp_div.h:127-201 -> op_div: long division with 32-bit quotient
- *            digits; digit estimate by the 3-by-2 division of the top three remainder
- *            words by the top two divisor words (exact up to +2), group
- *            multiply-subtract, add-back while negative.
+ *   div      fpa_warp_div.h:127-201 -> op_div: long division in radix 2^27 with the
+ *            remainder kept as lazy signed 64-bit column sums (one multiply-add per column
+ *            and digit), FP64 digit estimates, redundant digits, exact end correction.
  *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root with 32-bit
  *            digits in a fixed frame: R <- R*2^32 - d*(2S + d*2^(32 pos)), the digit
  *            estimated from (R/2)/S's top words, add-back while negative.
@@ -792,92 +791,6 @@ __host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gc
 /* digit steps per loop trip in div and sqrt (a divisor of K, so the trips cover n = K*L
  * steps) */
 __host__ __device__ constexpr int div_steps_per_trip(int K) { return gcd_int(K, FPA_DIV_UNROLL); }
-
-/* (re, Qr) = a / b; lds: n+2 words (quotient digits) */
-template <int K, int L>
-__device__ void core_div(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
-                         uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
-  static_assert(K >= 2, "the top lane must hold the two leading words after a shift");
-  constexpr int n = K * L;
-  uint32_t act = k_div_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
-  if (act != FPA_A_DIV) {
-    re = act;
-#pragma unroll
-    for (int i = 0; i < K; i++) Qr[i] = 0;
-    return;
-  }
-  uint32_t R[K];
-
-  const uint32_t sign = (ea ^ eb) & 1u;
-  int64_t E = (int64_t)(ea >> 1) - (int64_t)(eb >> 1) + FPA_BIAS_LL;
-  /* leading quotient bit: A >= B */
-  uint32_t below = g.sub(R, A, B);
-  const uint32_t topbit = below ? 0u : 1u;
-  if (below) {
-#pragma unroll
-    for (int i = 0; i < K; i++) R[i] = A[i];
-  }
-  const uint32_t d1 = g.template word<K, n - 1>(B), d0 = g.template word<K, n - 2>(B);
-  const uint32_t v = fpa_recip_3by2(d1, d0);
-
-  auto step = [&](int j) __attribute__((always_inline)) {
-    /* R <- R * 2^32; the word leaving the top becomes rtop (top lane only) */
-    int64_t rtop = R[K - 1];
-    uint32_t in = g.from_prev_row(R[K - 1]);
-#pragma unroll
-    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
-    R[0] = in;
-    /* the top lane holds the three leading words (rtop, R[K-1], R[K-2]) and estimates
-     * the digit; the other lanes' estimates are discarded by the broadcast */
-    const uint32_t u2 = (uint32_t)rtop, u1 = R[K - 1], u0 = R[K - 2];
-    uint32_t q;
-    if (u2 > d1 || (u2 == d1 && u1 >= d0)) q = 0xFFFFFFFFu;
-    else q = fpa_div3by2(u2, u1, u0, d1, d0, v);
-    q = g.bcast_top(q);
-    msub_step<K, L>(g, R, rtop, q, B, 0u);
-    if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
-      /* add back (the estimate exceeded the digit): rare, off the common path */
-      for (int guard = 0; top_negative(g, rtop) && guard < 4; guard++) {
-        uint32_t c = g.add(R, R, B);
-        rtop += c;
-        q -= 1u;
-      }
-    }
-    if (g.gl == 0) lds[n - j] = q;
-  };
-  /* U steps per trip: a step moves every word of R up one place, which in straight-line
-   * code is only a renaming, so R's registers rotate and the loop's back edge moves each
-   * word once per U steps instead of once per step */
-  constexpr int U = div_steps_per_trip(K);
-  int j0 = 0;
-  for (; j0 + U <= n + 1; j0 += U) repeat<U>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
-  repeat<(n + 1) % U>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
-  const uint32_t rem_nz = g.any(or_all(R) != 0) ? 1u : 0u;
-  wsync();
-  /* quotient digits Q[0..n] (Q[n] most significant), plus the leading bit */
-  uint32_t M[K];
-  uint32_t rw, sticky;
-  if (topbit) {
-#pragma unroll
-    for (int i = 0; i < K; i++) {
-      int w = g.gl * K + i;
-      uint32_t hi = (w + 2 <= n) ? lds[w + 2] : 1u;
-      M[i] = (lds[w + 1] >> 1) | (hi << 31);
-    }
-    rw = (lds[1] << 31) | (lds[0] >> 1);
-    sticky = (lds[0] & 1u) | rem_nz;
-    E += 1;
-  } else {
-#pragma unroll
-    for (int i = 0; i < K; i++) M[i] = lds[g.gl * K + i + 1];
-    rw = lds[0];
-    sticky = rem_nz;
-  }
-  wsync();
-  if (round_mantissa<K, L>(g, M, mode, sign, rw, sticky)) E += 1;
-  re = pack_exp(E, sign, 10, 0x7FFFFFFEll);
-  chunk_select(Qr, true, M, M);
-}
 
 /* ------------------- div with lazy radix-2^27 column sums ------------------- */
 /*
