@@ -217,6 +217,21 @@ def bench_config3(F, hip, nops, steps, cpu, threads):
     off = W.limb_offsets(precs)
     same = bool((be_ == pe_).all() and (bl_ == pl_).all())
     res["batch_equals_plain"] = same
+    # the same batch as one persistent kernel launch (tiles from a work queue), captured too
+    P = F.Batch(ops, precs, modes, persistent=True)
+    P.set_operands(ae, al, be, bl)
+    P.run()
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0)
+    for _ in range(steps):
+        P.run()
+    hip.record(e1)
+    ms = hip.elapsed_ms(e0, e1) / steps
+    xe_, xl_ = P.results()
+    res["persistent_kernel"] = {"ms": ms, "ops_per_s": nops / (ms / 1e3), "plan": P.info(),
+                                "equals_graph": bool((be_ == xe_).all() and (bl_ == xl_).all())}
+    P.close()
     if cpu:
         from oracle import oracle as O
         t_cpu, sample_ok, bad = 0.0, True, 0

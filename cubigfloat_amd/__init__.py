@@ -332,6 +332,12 @@ lib.fpa_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.PO
 lib.fpa_batch_set_operands.argtypes = [_vp, _u32p, _u32p, _u32p, _u32p]
 lib.fpa_batch_create_branches.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8),
                                           _u32p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32]
+lib.fpa_batch_create_persistent.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8),
+                                            _u32p, ctypes.POINTER(ctypes.c_uint8)]
+for _n in ("fpa_batch_tiles", "fpa_batch_grid"):
+    getattr(lib, _n).argtypes = [_vp]
+    getattr(lib, _n).restype = ctypes.c_uint32
+lib.fpa_batch_is_persistent.argtypes = [_vp]
 lib.fpa_batch_run.argtypes = [_vp, _vp]
 lib.fpa_batch_run_plain.argtypes = [_vp, _vp]
 lib.fpa_batch_get_results.argtypes = [_vp, _vp, _u32p, _u32p]
@@ -353,18 +359,24 @@ lib.fpa_dot_plan_destroy.argtypes = [_vp]
 lib.fpa_dot_plan_time.argtypes = [_vp, _P, _P, _P, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_float)]
 
 BATCH_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192, 16384)
+PERSIST_PRECISIONS = (256, 512, 1024, 2048, 4096, 8192)
+PERSIST_MAX_SEGMENTS = 40
 
 
 class Batch:
-    """A planned mixed-precision batch (fpa_batch_t): one captured fork/join graph."""
+    """A planned mixed-precision batch (fpa_batch_t): one captured fork/join graph, or with
+    persistent=True one captured launch of the persistent tile kernel."""
 
-    def __init__(self, ops, precisions, modes=None, branches=None):
+    def __init__(self, ops, precisions, modes=None, branches=None, persistent=False):
         ops = np.ascontiguousarray([OPS[o] if isinstance(o, str) else o for o in ops], dtype=np.uint8)
         precisions = np.ascontiguousarray(precisions, dtype=np.uint32)
         modes = np.ascontiguousarray(np.full(len(ops), ROUND_TIES_TO_EVEN) if modes is None else modes, dtype=np.uint8)
         self.h = _vp()
         u8 = ctypes.POINTER(ctypes.c_uint8)
-        if branches is None:
+        if persistent:
+            rc = lib.fpa_batch_create_persistent(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(u8),
+                                                 _ptr(precisions), modes.ctypes.data_as(u8))
+        elif branches is None:
             rc = lib.fpa_batch_create(ctypes.byref(self.h), len(ops), ops.ctypes.data_as(u8), _ptr(precisions),
                                       modes.ctypes.data_as(u8))
         else:
@@ -395,8 +407,11 @@ class Batch:
         return e, l
 
     def info(self):
-        return {"segments": lib.fpa_batch_segments(self.h), "branches": lib.fpa_batch_branches(self.h),
-                "blocks": lib.fpa_batch_blocks(self.h), "device_bytes": lib.fpa_batch_device_bytes(self.h)}
+        d = {"segments": lib.fpa_batch_segments(self.h), "branches": lib.fpa_batch_branches(self.h),
+             "blocks": lib.fpa_batch_blocks(self.h), "device_bytes": lib.fpa_batch_device_bytes(self.h)}
+        if lib.fpa_batch_is_persistent(self.h):
+            d.update(persistent=True, tiles=lib.fpa_batch_tiles(self.h), grid=lib.fpa_batch_grid(self.h))
+        return d
 
     def close(self):
         if self.h:

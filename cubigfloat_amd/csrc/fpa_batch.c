@@ -1,6 +1,7 @@
 /*
- * Host side of fpa_batch.h: batch planning (segments, one operand/result pool, hipGraph
- * capture of a fork/join graph of per-segment kernels) and dot-chain plans.
+ * Host side of fpa_batch.h: batch planning (segments, one operand/result pool), hipGraph
+ * capture of either a fork/join graph of per-segment kernels or one persistent kernel
+ * launch over cost-sorted tiles, and dot-chain plans.
  *
  * A batch becomes one graph: a fork node, `branches` independent chains of kernel nodes
  * and a join.  Every kernel node is the element-wise kernel of one (op, precision, mode)
@@ -29,6 +30,9 @@
   } while (0)
 
 int fpa_impl_dot_launch(const fpa_dot_arguments *d, int K, int L, void *stream);
+int fpa_impl_persist_kind(int op, uint32_t precision, int *K, int *L);
+int fpa_impl_persist_blocks_per_cu(void);
+int fpa_impl_persist_launch(const fpa_persist_arguments *P, uint32_t grid, void *stream);
 void *fpa_get_stream(void);
 
 #define FPA_BATCH_MAX_BRANCHES 8
@@ -54,6 +58,12 @@ struct fpa_batch_s {
   uint8_t *mirror;         /* pinned host copy of the pool layout */
   hipGraph_t graph;
   hipGraphExec_t exec;
+  /* persistent form: kernel arguments (segment table, tiles, queue), grid */
+  int persistent;
+  uint32_t ntiles, grid;
+  fpa_persist_arguments pargs;
+  uint32_t *tiles_dev;
+  uint32_t *queue_dev;
 };
 
 int fpa_batch_precision_supported(uint32_t precision) { return fpa_precision_supported(precision); }
@@ -82,13 +92,20 @@ static int launch_segment(const seg_host *g, void *stream) {
   return fpa_kernel_launch(g->op, &g->args, g->variant, stream);
 }
 
-fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
-                                     const uint32_t *precisions, const uint8_t *modes, uint32_t branches) {
+/* Segments keyed by (op, precision, mode) in first-appearance order, the device pool
+ * (per segment: results, a, b) and its pinned host mirror, each segment's kernel
+ * arguments.  shape(op, precision, count, &K, &L) gives a segment's shape and returns
+ * its variant or kind (-1: unsupported precision). */
+typedef int (*shape_fn)(int op, uint32_t precision, uint32_t count, int *K, int *L);
+
+static fpa_result plan_batch(struct fpa_batch_s **out, uint32_t nops, const uint8_t *ops, const uint32_t *precisions,
+                             const uint8_t *modes, shape_fn shape) {
   *out = NULL;
-  if (branches < 1 || branches > FPA_BATCH_MAX_BRANCHES) return fpa_invalid_counts;
   for (uint32_t i = 0; i < nops; i++) {
     if (ops[i] < FPA_OP_ADD || ops[i] > FPA_OP_SQRT || modes[i] > 4) return fpa_math_error;
     if (!fpa_batch_precision_supported(precisions[i])) return fpa_invalid_precisions;
+    int K, L;
+    if (shape(ops[i], precisions[i], 1, &K, &L) < 0) return fpa_invalid_precisions;
   }
   struct fpa_batch_s *b = (struct fpa_batch_s *)calloc(1, sizeof *b);
   b->nops = nops;
@@ -97,7 +114,6 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
   b->op_off = (uint64_t *)malloc(sizeof(uint64_t) * (nops + 1));
   int cap = 16;
   b->sh = (seg_host *)calloc(cap, sizeof(seg_host));
-  /* segments keyed by (op, precision, mode), in first-appearance order */
   uint64_t off = 0;
   for (uint32_t i = 0; i < nops; i++) {
     uint32_t s;
@@ -132,13 +148,7 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
   FPA_HIP(hipHostMalloc((void **)&b->mirror, b->pool_bytes, hipHostMallocDefault));
   for (uint32_t s = 0; s < b->nseg; s++) {
     seg_host *g = &b->sh[s];
-    /* A segment shares the chip with the branches running beside it, so it takes the shape
-     * tuned for a launch twice its size: on the 2^18-op mixed batch this beat the shape for
-     * its own size (the isolated-latency winner, which takes chip share from the other
-     * branches), four times its size and the large-launch shapes (0.445 ms against 0.50,
-     * 0.49 and 0.57 ms). */
-    g->variant = fpa_kernel_get_variant_for_count(g->op, g->precision, 2 * (uint64_t)g->count);
-    fpa_kernel_variant_shape(g->op, g->precision, g->variant, &g->K, &g->L);
+    g->variant = shape(g->op, g->precision, g->count, &g->K, &g->L);
     fpa_arguments *a = &g->args;
     a->count = g->count;
     a->precision = g->precision;
@@ -154,6 +164,33 @@ fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint
     b->blocks += g->blocks;
     g->cost = (double)g->blocks * 4 * wave_cost(g->op, g->K, g->L);
   }
+  *out = b;
+  return fpa_success;
+}
+
+/* A segment of the graph form shares the chip with the branches running beside it, so it
+ * takes the shape tuned for a launch twice its size: on the 2^18-op mixed batch this beat
+ * the shape for its own size (the isolated-latency winner, which takes chip share from the
+ * other branches), four times its size and the large-launch shapes (0.445 ms against 0.50,
+ * 0.49 and 0.57 ms). */
+static int graph_shape(int op, uint32_t precision, uint32_t count, int *K, int *L) {
+  int v = fpa_kernel_get_variant_for_count(op, precision, 2 * (uint64_t)count);
+  if (v < 0 || fpa_kernel_variant_shape(op, precision, v, K, L) != 0) return -1;
+  return v;
+}
+
+static int persist_shape(int op, uint32_t precision, uint32_t count, int *K, int *L) {
+  (void)count;
+  return fpa_impl_persist_kind(op, precision, K, L);
+}
+
+fpa_result fpa_batch_create_branches(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
+                                     const uint32_t *precisions, const uint8_t *modes, uint32_t branches) {
+  *out = NULL;
+  if (branches < 1 || branches > FPA_BATCH_MAX_BRANCHES) return fpa_invalid_counts;
+  struct fpa_batch_s *b;
+  fpa_result pr = plan_batch(&b, nops, ops, precisions, modes, graph_shape);
+  if (pr != fpa_success) return pr;
   /* longest processing time first onto the least-loaded branch */
   b->order = (uint32_t *)malloc(sizeof(uint32_t) * (b->nseg ? b->nseg : 1));
   seg_cost *sc = (seg_cost *)malloc(sizeof(seg_cost) * (b->nseg ? b->nseg : 1));
@@ -202,6 +239,77 @@ fpa_result fpa_batch_create(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
   return fpa_batch_create_branches(out, nops, ops, precisions, modes, FPA_BATCH_DEFAULT_BRANCHES);
 }
 
+typedef struct { double cost; uint32_t seg, first; } tile_host;
+static int tile_cmp(const void *x, const void *y) {
+  const tile_host *a = (const tile_host *)x, *b = (const tile_host *)y;
+  if (a->cost != b->cost) return a->cost > b->cost ? -1 : 1;
+  if (a->seg != b->seg) return a->seg < b->seg ? -1 : 1;
+  return a->first < b->first ? -1 : (a->first > b->first);
+}
+
+fpa_result fpa_batch_create_persistent(fpa_batch_t *out, uint32_t nops, const uint8_t *ops,
+                                       const uint32_t *precisions, const uint8_t *modes) {
+  struct fpa_batch_s *b;
+  fpa_result pr = plan_batch(&b, nops, ops, precisions, modes, persist_shape);
+  if (pr != fpa_success) return pr;
+  b->persistent = 1;
+  b->branches = 1;
+  if (b->nseg > FPA_PERSIST_MAX_SEGMENTS) {
+    fpa_batch_destroy(b);
+    return fpa_invalid_counts;
+  }
+  /* the segment table, carried in the kernel arguments */
+  memset(&b->pargs, 0, sizeof b->pargs);
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    b->pargs.seg[s] = b->sh[s].args;
+    b->pargs.kind[s] = (uint32_t)b->sh[s].variant;
+  }
+  /* tiles: one wavefront's values of one segment, most expensive first */
+  uint64_t nt = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) nt += (b->sh[s].count + (64 / b->sh[s].L) - 1) / (64 / b->sh[s].L);
+  tile_host *th = (tile_host *)malloc(sizeof(tile_host) * (nt ? nt : 1));
+  uint64_t k = 0;
+  for (uint32_t s = 0; s < b->nseg; s++) {
+    const seg_host *g = &b->sh[s];
+    const uint32_t vpw = 64 / g->L;
+    const double c = wave_cost(g->op, g->K, g->L);
+    for (uint32_t f = 0; f < g->count; f += vpw) { th[k].cost = c; th[k].seg = s; th[k].first = f; k++; }
+  }
+  qsort(th, nt, sizeof(tile_host), tile_cmp);
+  uint32_t *tt = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (nt ? nt : 1));
+  for (uint64_t i = 0; i < nt; i++) { tt[2 * i] = th[i].seg; tt[2 * i + 1] = th[i].first; }
+  free(th);
+  b->ntiles = (uint32_t)nt;
+  FPA_HIP(hipMalloc((void **)&b->tiles_dev, sizeof(uint32_t) * 2 * (nt ? nt : 1)));
+  FPA_HIP(hipMemcpy(b->tiles_dev, tt, sizeof(uint32_t) * 2 * nt, hipMemcpyHostToDevice));
+  free(tt);
+  /* the work queue: next tile, waves done; zero here, and set back to zero by the last
+   * wave of every run */
+  FPA_HIP(hipMalloc((void **)&b->queue_dev, 256));
+  FPA_HIP(hipMemset(b->queue_dev, 0, 256));
+  b->pargs.tiles = b->tiles_dev;
+  b->pargs.queue = b->queue_dev;
+  b->pargs.ntiles = b->ntiles;
+  /* grid: every resident block slot of the device, no more blocks than the tiles need */
+  int dev = 0, cus = 0;
+  FPA_HIP(hipGetDevice(&dev));
+  FPA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint64_t grid = (uint64_t)cus * fpa_impl_persist_blocks_per_cu(), need = (nt + 3) / 4;
+  b->grid = (uint32_t)(grid < need ? grid : (need ? need : 1));
+  FPA_HIP(hipDeviceSynchronize());
+  /* capture the one launch */
+  hipStream_t cs;
+  FPA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  FPA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = fpa_impl_persist_launch(&b->pargs, b->grid, cs);
+  FPA_HIP(hipStreamEndCapture(cs, &b->graph));
+  if (rc != 0) { fprintf(stderr, "persistent batch capture failed: kernel launch returned %d\n", rc); exit(1); }
+  FPA_HIP(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+  FPA_HIP(hipStreamDestroy(cs));
+  *out = b;
+  return fpa_success;
+}
+
 static void scatter_operand(struct fpa_batch_s *b, int which, const uint32_t *exps, const uint32_t *limbs) {
   for (uint32_t i = 0; i < b->nops; i++) {
     seg_host *g = &b->sh[b->op_seg[i]];
@@ -238,6 +346,11 @@ fpa_result fpa_batch_run(fpa_batch_t b, void *stream) {
 }
 
 fpa_result fpa_batch_run_plain(fpa_batch_t b, void *stream) {
+  if (b->persistent) {
+    int rc = fpa_impl_persist_launch(&b->pargs, b->grid, stream);
+    if (rc != 0) { fprintf(stderr, "persistent batch launch failed: %d\n", rc); exit(1); }
+    return fpa_success;
+  }
   for (uint32_t k = 0; k < b->nseg; k++) {
     int rc = launch_segment(&b->sh[b->order[k]], stream);
     if (rc != 0) { fprintf(stderr, "plain launch of segment %u failed: %d\n", b->order[k], rc); exit(1); }
@@ -264,10 +377,14 @@ fpa_result fpa_batch_get_results(fpa_batch_t b, void *stream, uint32_t *r_exps, 
 
 fpa_result fpa_batch_destroy(fpa_batch_t b) {
   if (!b) return fpa_success;
-  FPA_HIP(hipGraphExecDestroy(b->exec));
-  FPA_HIP(hipGraphDestroy(b->graph));
+  if (b->exec) FPA_HIP(hipGraphExecDestroy(b->exec));
+  if (b->graph) FPA_HIP(hipGraphDestroy(b->graph));
   FPA_HIP(hipFree(b->pool));
   FPA_HIP(hipHostFree(b->mirror));
+  if (b->persistent) {
+    if (b->tiles_dev) FPA_HIP(hipFree(b->tiles_dev));
+    if (b->queue_dev) FPA_HIP(hipFree(b->queue_dev));
+  }
   free(b->op_seg);
   free(b->op_slot);
   free(b->op_off);
@@ -282,6 +399,9 @@ uint32_t fpa_batch_branches(fpa_batch_t b) { return b->branches; }
 uint64_t fpa_batch_blocks(fpa_batch_t b) { return b->blocks; }
 size_t fpa_batch_device_bytes(fpa_batch_t b) { return b->pool_bytes; }
 uint64_t fpa_batch_total_limbs(fpa_batch_t b) { return b->total_limbs; }
+int fpa_batch_is_persistent(fpa_batch_t b) { return b->persistent; }
+uint32_t fpa_batch_tiles(fpa_batch_t b) { return b->ntiles; }
+uint32_t fpa_batch_grid(fpa_batch_t b) { return b->grid; }
 
 /* ------------------------------ dot chains ------------------------------ */
 struct fpa_dot_plan_s {

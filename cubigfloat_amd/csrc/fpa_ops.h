@@ -138,7 +138,7 @@ __device__ __forceinline__ void load_value(const Group<L> &g, const uint32_t *li
 
 /* ------------------------------- neg ------------------------------- */
 template <int K, int L>
-__device__ void op_neg(const fpa_arguments &a, const Slot &s, const Group<L> &g) {
+__device__ __forceinline__ void op_neg(const fpa_arguments &a, const Slot &s, const Group<L> &g) {
   uint32_t ea = a.a_exps[s.v];
   uint32_t act = k_neg_tab[fpa_classify(ea)];
   uint32_t x[K];
@@ -149,7 +149,7 @@ __device__ void op_neg(const fpa_arguments &a, const Slot &s, const Group<L> &g)
 /* ----------------------------- add / sub ----------------------------- */
 /* (re, R) = a +- b for the group's values (ea, A), (eb, B); lds: >= n+1 words */
 template <int K, int L>
-__device__ void core_addsub(const Group<L> &g, uint32_t *lds, int mode, bool is_sub, uint32_t ea,
+__device__ __forceinline__ void core_addsub(const Group<L> &g, uint32_t *lds, int mode, bool is_sub, uint32_t ea,
                             const uint32_t (&A)[K], uint32_t eb, const uint32_t (&B)[K], uint32_t &re,
                             uint32_t (&R)[K]) {
   constexpr int n = K * L;
@@ -279,7 +279,7 @@ __device__ void core_addsub(const Group<L> &g, uint32_t *lds, int mode, bool is_
 }
 
 template <int K, int L>
-__device__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, bool is_sub) {
+__device__ __forceinline__ void op_addsub(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, bool is_sub) {
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
@@ -494,7 +494,7 @@ __device__ __forceinline__ uint32_t add_carry_digits_wave(uint32_t (&dg)[KD], ui
  * correct for it once.  Without it (dot chains: groups run chains of different lengths)
  * the top lane's incoming value is masked each round instead. */
 template <int K, int L, bool kLockstep>
-__device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int mode, uint32_t ea,
+__device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int mode, uint32_t ea,
                          const uint32_t (&Aw)[K], uint32_t eb, const uint32_t (&Bw)[K], uint32_t &re,
                          uint32_t (&R)[K]) {
   constexpr int n = K * L;
@@ -747,7 +747,7 @@ __device__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int
 }
 
 template <int K, int L>
-__device__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, int group_stride) {
+__device__ __forceinline__ void op_mul(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds, int group_stride) {
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
@@ -914,7 +914,7 @@ __device__ __forceinline__ int64_t normalize_exact(const Group<L> &g, int64_t (&
 
 /* (re, Qr) = a / b; lds: div_lds_words(K, L) words */
 template <int K, int L>
-__device__ void core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+__device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                               uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
   constexpr int n = K * L, p = 32 * n;
   constexpr int KD = div_digits_per_lane(K), ND = KD * L, S = div_steps(n);
@@ -1069,7 +1069,7 @@ __device__ void core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32
 }
 
 template <int K, int L>
-__device__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
+__device__ __forceinline__ void op_div(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);

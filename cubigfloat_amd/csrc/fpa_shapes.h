@@ -30,4 +30,17 @@
  * real call, which no kernel of this library may contain. */
 #define FPA_DOT_MAX_K 20
 
+/* Kinds of the persistent batch kernel: X(op, n, K, L), op 1 add, 2 sub, 3 mul, 4 div,
+ * 5 sqrt, one shape per (op, precision) for 256 ... 8192 bits.  Every kind is inlined into
+ * one kernel whose registers are the largest kind's, so the shapes are the tuned ones held
+ * to at most 128 VGPRs (4 waves per SIMD): div takes 8 limbs per lane, where its 16-limb
+ * shapes need up to 164. */
+#define FPA_PERSIST_SHAPES(X)                                                                \
+  X(1, 8, 8, 1)   X(2, 8, 8, 1)   X(3, 8, 8, 1)   X(4, 8, 8, 1)   X(5, 8, 8, 1)              \
+  X(1, 16, 8, 2)  X(2, 16, 8, 2)  X(3, 16, 8, 2)  X(4, 16, 8, 2)  X(5, 16, 8, 2)             \
+  X(1, 32, 8, 4)  X(2, 32, 8, 4)  X(3, 32, 8, 4)  X(4, 32, 8, 4)  X(5, 32, 8, 4)             \
+  X(1, 64, 8, 8)  X(2, 64, 8, 8)  X(3, 64, 8, 8)  X(4, 64, 8, 8)  X(5, 64, 16, 4)            \
+  X(1, 128, 8, 16) X(2, 128, 8, 16) X(3, 128, 16, 8) X(4, 128, 8, 16) X(5, 128, 16, 8)       \
+  X(1, 256, 8, 32) X(2, 256, 8, 32) X(3, 256, 16, 16) X(4, 256, 8, 32) X(5, 256, 16, 16)
+
 #endif

@@ -32,6 +32,19 @@ extern "C" {
 /* parallel branches of a batch graph made by fpa_batch_create (1..8 allowed) */
 #define FPA_BATCH_DEFAULT_BRANCHES 4
 
+/* kernel arguments of a persistent batch: the segment table (each segment's arguments and
+ * its kind, the (op, precision) shape compiled into the kernel), the cost-sorted tiles
+ * (segment, first value) and the work queue (next tile, waves done), in device memory */
+#define FPA_PERSIST_MAX_SEGMENTS 40
+typedef struct {
+  fpa_arguments   seg[FPA_PERSIST_MAX_SEGMENTS];
+  uint32_t        kind[FPA_PERSIST_MAX_SEGMENTS];
+  const uint32_t *tiles;
+  uint32_t       *queue;
+  uint32_t        ntiles;
+  uint32_t        pad;
+} fpa_persist_arguments;
+
 /* device-side arguments of the dot-chain kernel */
 typedef struct {
   uint32_t        count;       /* number of chains */
@@ -56,6 +69,20 @@ fpa_result fpa_batch_create(fpa_batch_t *batch, uint32_t nops, const uint8_t *op
 /* Same with an explicit branch count (1..8; fpa_invalid_counts otherwise). */
 fpa_result fpa_batch_create_branches(fpa_batch_t *batch, uint32_t nops, const uint8_t *ops,
                                      const uint32_t *precisions, const uint8_t *modes, uint32_t branches);
+/* The same batch as one persistent kernel launch captured in a hipGraph: waves take
+ * one-wave tiles of the segments from a self-resetting work queue, most expensive first,
+ * each with its (op, precision) kind's shape (precisions 256, 512, 1024, 2048, 4096 and
+ * 8192 only; fpa_invalid_precisions otherwise; at most FPA_PERSIST_MAX_SEGMENTS (op,
+ * precision, mode) segments, fpa_invalid_counts otherwise).  Results equal
+ * fpa_batch_create's bit for bit.  Runs of one batch must not overlap (they share its
+ * pool and queue). */
+fpa_result fpa_batch_create_persistent(fpa_batch_t *batch, uint32_t nops, const uint8_t *ops,
+                                       const uint32_t *precisions, const uint8_t *modes);
+/* 1 if the batch runs as one persistent kernel; its tiles and grid (blocks) */
+int fpa_batch_is_persistent(fpa_batch_t batch);
+uint32_t fpa_batch_tiles(fpa_batch_t batch);
+uint32_t fpa_batch_grid(fpa_batch_t batch);
+
 /* Upload operands packed in op order (b may be NULL if no op is binary). */
 fpa_result fpa_batch_set_operands(fpa_batch_t batch, const uint32_t *a_exps, const uint32_t *a_limbs,
                                   const uint32_t *b_exps, const uint32_t *b_limbs);

@@ -151,3 +151,65 @@ def test_dot_config4_shape_sample():
     re, rl = F.dot_chains(2048, lengths, xe, xl, ye, yl)
     me, ml = O.mpfr_dot(2048, lengths, xe, xl, ye, yl)
     assert O.results_equal(re, rl, me, ml, 2048).all()
+
+
+# ------------------------- persistent batch (one kernel launch) -------------------------
+def test_persistent_batch_vs_oracle_many_tiles_per_wave():
+    """The persistent kernel on a batch with far more tiles than waves (so waves take many
+    tiles from the queue), every replay checked, eager launches between replays."""
+    ops, precs, modes = W.mixed_ops(24000, 11, precisions=F.PERSIST_PRECISIONS, modes=(3,))
+    ae, al, be, bl = W.mixed_operands(ops, precs, 12)
+    say(f"[gpu] persistent batch nops={len(ops)}")
+    B = F.Batch(ops, precs, modes, persistent=True)
+    info = B.info()
+    say(f"[gpu] persistent plan {info}")
+    assert info["persistent"] and info["tiles"] > 4 * info["grid"]
+    B.set_operands(ae, al, be, bl)
+    oe, ol = oracle_packed(ops, precs, modes, ae, al, be, bl)
+    A = F.FpaArray(4096, 1024)
+    try:
+        for rep in range(4):
+            assert B.run() == F.SUCCESS
+            re, rl = B.results()
+            bad = compare_packed(precs, re, rl, oe, ol)
+            assert not bad, f"replay {rep}: {len(bad)} mismatches, first op {bad[0]}: op={INV[int(ops[bad[0]])]} p={precs[bad[0]]}"
+            assert F.mul(A, A, A) == F.SUCCESS      # eager work between replays
+        assert B.run_plain() == F.SUCCESS
+        re, rl = B.results()
+        assert not compare_packed(precs, re, rl, oe, ol)
+    finally:
+        A.close()
+        B.close()
+
+
+def test_persistent_batch_equals_graph_batch_all_modes():
+    ops, precs, modes = W.mixed_ops(6000, 13, precisions=(256, 1024, 4096), modes=(1, 4))
+    ae, al, be, bl = W.mixed_operands(ops, precs, 14)
+    say(f"[gpu] persistent vs graph nops={len(ops)}")
+    G = F.Batch(ops, precs, modes)
+    P = F.Batch(ops, precs, modes, persistent=True)
+    try:
+        for b in (G, P):
+            b.set_operands(ae, al, be, bl)
+            assert b.run() == F.SUCCESS
+        ge, gl = G.results()
+        pe, pl = P.results()
+        assert np.array_equal(ge, pe) and np.array_equal(gl, pl)
+        oe, ol = oracle_packed(ops, precs, modes, ae, al, be, bl)
+        assert not compare_packed(precs, pe, pl, oe, ol)
+    finally:
+        G.close()
+        P.close()
+
+
+def test_persistent_batch_limits():
+    ops = np.array([F.OPS["mul"]] * 4, np.uint8)
+    with pytest.raises(ValueError):
+        F.Batch(ops, np.array([16384] * 4, np.uint32), persistent=True)      # no kind for 16384 bits
+    n = F.PERSIST_MAX_SEGMENTS + 1
+    ops2 = np.array([F.OPS[o] for o in ("add", "sub", "mul", "div", "sqrt")] * 9, np.uint8)[:n]
+    precs2 = np.array([p for p in F.PERSIST_PRECISIONS for _ in range(5)] * 2, np.uint32)[:n]
+    modes2 = np.array(([3] * 30 + [0] * 30), np.uint8)[:n]
+    assert len(set(zip(ops2.tolist(), precs2.tolist(), modes2.tolist()))) == n
+    with pytest.raises(ValueError):
+        F.Batch(ops2, precs2, modes2, persistent=True)                       # more segments than the table holds
