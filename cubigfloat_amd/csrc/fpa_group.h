@@ -296,33 +296,47 @@ struct Group {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((gbase + src) << 2, (int)x);
   }
 
-  /* x as held by the group's top lane.  Groups of 2 and 4 lanes sit inside one DPP quad, so
-   * the broadcast is one VALU move (quad_perm [1,1,3,3] or [3,3,3,3]) instead of a
-   * ds_bpermute round trip: it sits on the serial path of every div and sqrt digit step.
-   * Groups of 8 take each quad's lane 3, then the lower quad takes the upper quad's
-   * (row_shl:4; the row's last quad reads past the row, a value it does not use). */
+  /* x as held by the group's top lane, through DPP moves instead of a ds_bpermute round
+   * trip: it sits on the serial path of every div and sqrt digit step.
+   *   L = 2, 4:  one quad_perm ([1,1,3,3] or [3,3,3,3]).
+   *   L = 16:    one row_newbcast:15 (lane 15 of each 16-lane row to the whole row).
+   *   L = 8:     row_newbcast:15 to the whole row, then row_newbcast:7 written only to
+   *              banks 0 and 1 (row lanes 0-7; the other lanes keep the first move's value).
+   *   L = 32:    ds_bpermute (two readlanes and a select measured 4-5% slower in sqrt). */
   __device__ __forceinline__ uint32_t bcast_top(uint32_t x) const {
     if constexpr (L == 1) return x;
     else if constexpr (L == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xF5, 0xF, 0xF, false);
     else if constexpr (L == 4) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xFF, 0xF, 0xF, false);
     else if constexpr (L == 8) {
-      const uint32_t t = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xFF, 0xF, 0xF, false);
-      const uint32_t u = (uint32_t)__builtin_amdgcn_mov_dpp((int)t, 0x104, 0xF, 0xF, true);
-      return gl < 4 ? u : t;
+      const int t = __builtin_amdgcn_mov_dpp((int)x, 0x15F, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_update_dpp(t, (int)x, 0x157, 0xF, 0x3, false);
+    } else if constexpr (L == 16) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x15F, 0xF, 0xF, false);
     } else return bcast(x, L - 1);
   }
 
-  /* from_prev through a DPP row shift (row_shr:1) for groups inside one 16-lane row: a
-   * VALU move instead of a ds_bpermute round trip, for the digit steps' serial path.  Lane
-   * 0 of a group takes 'fill' (the row shift gave it the previous group's last lane, or
-   * zero at the row's start). */
+  /* from_prev through a DPP move for the digit steps' serial path, instead of a ds_bpermute
+   * round trip: a row shift (row_shr:1) for groups inside one 16-lane row, a wave shift
+   * (wave_shr:1) for groups of 32.  Lane 0 of a group takes 'fill' (the shift gave it the
+   * previous group's last lane, or zero at the row's or wave's start). */
   __device__ __forceinline__ uint32_t from_prev_row(uint32_t x, uint32_t fill = 0) const {
     if constexpr (L >= 2 && L <= 16) {
       const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x111, 0xF, 0xF, true);
       return gl == 0 ? fill : y;
+    } else if constexpr (L == 32) {
+      /* wave_shr:1 moves every lane's value one lane up across row boundaries */
+      const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x138, 0xF, 0xF, true);
+      return gl == 0 ? fill : y;
     } else {
       return from_prev(x, fill);
     }
+  }
+
+  /* from_prev_row with a zero fill: a 16-lane group is a whole DPP row, whose row_shr:1
+   * already gives its lane 0 a zero (bound_ctrl), so no select is needed there */
+  __device__ __forceinline__ uint32_t from_prev_row0(uint32_t x) const {
+    if constexpr (L == 16) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x111, 0xF, 0xF, true);
+    else return from_prev_row(x, 0u);
   }
 
   /* x of group lane gl-1; 'fill' in group lane 0 */

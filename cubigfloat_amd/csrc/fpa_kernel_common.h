@@ -14,14 +14,14 @@
  * lane needs 129-140 VGPRs left alone, which allows 3 waves per SIMD; held to 128 it
  * does not spill and runs 4.  Others take the compiler's own choice (1 = no target),
  * unless a build sets one for every kernel (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
-__host__ __device__ constexpr int fpa_waves_per_eu(int op, int K) {
+__host__ __device__ constexpr int fpa_waves_per_eu(int op, int K, int L) {
 #ifdef FPA_WAVES_PER_EU
-  return (void)op, (void)K, FPA_WAVES_PER_EU;
+  return (void)op, (void)K, (void)L, FPA_WAVES_PER_EU;
 #else
-  return op == 5 && K == 32 ? 4 : 1;
+  return op == 5 && K == 32 ? 4 : op == 4 && K == 16 && L == 8 ? 4 : 1;
 #endif
 }
-#define FPA_KERNEL_ATTRS(OP, K) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K))))
+#define FPA_KERNEL_ATTRS(OP, K, L) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K, L))))
 
 namespace fpa {
 
@@ -39,7 +39,7 @@ struct ScratchWords {
 };
 
 template <int K, int L, int OP>
-__global__ FPA_KERNEL_ATTRS(OP, K) void fpa_kernel(fpa_arguments a) {
+__global__ FPA_KERNEL_ATTRS(OP, K, L) void fpa_kernel(fpa_arguments a) {
   constexpr int VPW = 64 / L;
   constexpr int SW = ScratchWords<OP, K, L>::value;
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];

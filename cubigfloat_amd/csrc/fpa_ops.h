@@ -773,7 +773,7 @@ __device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], i
     bw = o;
   }
   uint32_t out = c + bw;                     /* <= 2^32-1: high product word is <= 2^32-2 */
-  uint32_t bout = g.sub_lane_words(R, g.from_prev_row(out));
+  uint32_t bout = g.sub_lane_words(R, g.from_prev_row0(out));
   rtop = rtop - (int64_t)out - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
 }
 
@@ -803,15 +803,18 @@ __host__ __device__ constexpr int div_steps_per_trip(int K) { return gcd_int(K, 
  *      the top is folded into the new top column (c_top * beta + c, exact modulo 2^64,
  *      and small in truth because the remainder stays below 2D);
  *   2. the group's top lane estimates the digit q = floor(V / Dt) in FP64 from its top two
- *      columns V = c1 * beta + c0 and the divisor's top two digits Dt (the neglected
- *      columns below contribute less than 2^-17 of a digit), and broadcasts it;
+ *      columns V = c1 * beta + c0 and the divisor's top two digits Dt in [2^53, 2^54), and
+ *      broadcasts it.  Only c0's high word enters (one 32-bit conversion): that drops less
+ *      than 2^32 of V, under 2^-21 of a digit, and the columns below c0 add less than 2^-17
+ *      of one;
  *   3. every lane subtracts q * d from its columns: one v_mad_i64_i32 per column with the
  *      negated divisor digits.
- * The estimate can be one too large or too small, so the digits are redundant (signed,
- * |q| < 2 beta) and the remainder stays in (-D, 2D); |q * d| < 2^55, so a column that lives
- * ND <= 255 steps stays below 2^63, and longer values renormalise every 128 steps by one
- * carry level.  At the end, the FP64 estimate of R/D decides (R in (0, D): no correction,
- * remainder nonzero) unless it lies within 2^-16 of 0 or 1; then R is normalised exactly
+ * The estimate lies within 2^-16 of the true quotient x, so q = floor of it leaves x - q in
+ * (-2^-16, 1 + 2^-16): the digits are redundant (signed, |q| < beta + 2^12) and the
+ * remainder stays in (-D, 2D); |q * d| < 2^55, so a column that lives ND <= 255 steps stays
+ * below 2^63, and longer values renormalise every 128 steps by one carry level.  At the
+ * end, the FP64 estimate of R/D decides (R in (0, D): no correction, remainder nonzero)
+ * unless it lies within 2^-16 of 0 or 1; then R is normalised exactly
  * and corrected by one D (quotient -1 or +1).  The quotient digits (one per step, in LDS)
  * are carry-normalised only when some digit lies outside [0, beta).  The reference's
  * division (fpa_warp_div.h:127-201) runs 32-bit digits with exact remainders instead.
@@ -946,8 +949,10 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
   int64_t C[KD];
 #pragma unroll
   for (int i = 0; i < KD; i++) { nd[i] = -(int32_t)dd[i]; C[i] = (int64_t)rd[i]; }
-  /* divisor's top two digits (meaningful in the top lane) */
+  /* divisor's top two digits (meaningful in the top lane); inv32 = 2^32 / Dt scales the
+   * step estimate, which works on V / 2^32 */
   const double inv = 1.0 / __builtin_fma((double)dd[KD - 1], (double)BETA, (double)dd[KD - 2]);
+  const double inv32 = inv * 0x1p32;
   const bool top = g.gl == L - 1;
   int32_t *qd = reinterpret_cast<int32_t *>(lds);   /* quotient digit of step j at S-1-j */
 
@@ -955,15 +960,16 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     /* R <- R * beta: columns move up; the column leaving a lane enters the next lane's
      * bottom; the one leaving the top lane folds into the new top column */
     const int64_t so = C[KD - 1];
-    const uint32_t inlo = g.from_prev_row((uint32_t)so), inhi = g.from_prev_row((uint32_t)((uint64_t)so >> 32));
+    const uint32_t inlo = g.from_prev_row0((uint32_t)so), inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
 #pragma unroll
     for (int i = KD - 1; i > 0; i--) C[i] = C[i - 1];
     C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
     const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
     C[KD - 1] = (int64_t)((uint64_t)C[KD - 1] + fold);
-    /* digit estimate from the top two columns (top lane), broadcast */
-    const double V = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2]));
-    int32_t q = (int32_t)__builtin_floor(V * inv);
+    /* digit estimate from the top two columns (top lane), broadcast: V / 2^32 from
+     * c1 * 2^-5 + floor(c0 / 2^32) */
+    const double V = __builtin_fma(i64_to_f64(C[KD - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KD - 2] >> 32));
+    int32_t q = (int32_t)__builtin_floor(V * inv32);
     q = (int32_t)g.bcast_top((uint32_t)q);
 #pragma unroll
     for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nd[i] + C[i];
@@ -978,8 +984,8 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
         int64_t car[KD];
 #pragma unroll
         for (int i = 0; i < KD; i++) car[i] = C[i] >> DIV_RADIX;
-        const uint32_t clo = g.from_prev_row((uint32_t)car[KD - 1]);
-        const uint32_t chi = g.from_prev_row((uint32_t)((uint64_t)car[KD - 1] >> 32));
+        const uint32_t clo = g.from_prev_row0((uint32_t)car[KD - 1]);
+        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KD - 1] >> 32));
         const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
 #pragma unroll
         for (int i = KD - 1; i >= 0; i--) {
