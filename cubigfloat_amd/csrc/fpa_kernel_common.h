@@ -31,7 +31,7 @@ struct ScratchWords {
   static constexpr int n = K * L;
   static constexpr int raw = OP == 0 ? 0
                            : OP <= 2 ? n + 1
-                           : OP == 3 ? mul_lds_words(K, L)
+                           : OP == 3 ? mul_op_lds_words<K, L>()
                            : OP == 4 ? div_lds_words(K, L)
                            : n + 1;
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */

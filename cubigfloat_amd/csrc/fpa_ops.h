@@ -24,6 +24,7 @@
 #define FPA_OPS_H
 
 #include "fpa_group.h"
+#include "fpa_mul_sched.h"
 #include "../../include/fpa_kernels.h"
 
 namespace fpa {
@@ -485,6 +486,12 @@ __device__ __forceinline__ uint32_t add_carry_digits_wave(uint32_t (&dg)[KD], ui
   return c32;
 }
 
+template <int K, int L, bool kShort>
+__device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
+                                           const uint64_t (&cl)[mul_digits_per_lane(K)],
+                                           const uint64_t (&ch)[mul_digits_per_lane(K)], int mode, uint32_t ea,
+                                           uint32_t eb, uint32_t &re, uint32_t (&R)[K], bool &amb);
+
 /* (re, R) = a * b; lds: mul_lds_words(K, L) words (64-bit low columns, then digits; a's
  * digit chunks in their top half while the rounds run).  group_stride = LDS words from this group's
  * base to the next group's in the same wave.
@@ -497,11 +504,9 @@ template <int K, int L, bool kLockstep>
 __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int group_stride, int mode, uint32_t ea,
                          const uint32_t (&Aw)[K], uint32_t eb, const uint32_t (&Bw)[K], uint32_t &re,
                          uint32_t (&R)[K]) {
-  constexpr int n = K * L;
   constexpr int KD = mul_digits_per_lane(K);
   constexpr int KDP = mul_digits_padded(K);
   constexpr int ND = KD * L;               /* digits per operand */
-  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
   static_assert(2 * L <= 64, "low and high chunk carries share one 64-bit mask");
   /* a column sums at most ND products below 2^54; on the lockstep row path a window can
    * also hold one chunk of the neighbour group on top of its own */
@@ -618,14 +623,41 @@ __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int g
     }
   }
 
+  uint64_t cl[KD], ch[KD];
+  const uint64_t *lrow = low + g.gl * KD;     /* one base address, constant offsets */
+#pragma unroll
+  for (int i = 0; i < KD; i++) { cl[i] = lrow[i]; ch[i] = W[i]; }
+  bool amb;
+  mul_finish<K, L, false>(g, lds, cl, ch, mode, ea, eb, re, R, amb);
+  if (kLockstep && act != FPA_A_MUL) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = 0;
+  }
+}
+
+/* Columns to the rounded product.  cl / ch: this lane's low and high chunk of column
+ * sums (columns gl*KD + i and ND + gl*KD + i).  The columns are normalised to radix-2^27
+ * digits, the mantissa words and round word are extracted and the result is rounded.
+ * kShort: the columns hold a short product Q with Q <= A*B < Q + 2^(27*T + 36) (the
+ * omitted digit products all lie in columns below T, mul_short); the rounding is then
+ * decided by Q alone, with sticky 1, unless the 64 bits of Q just below the round word
+ * are all zeros or all ones: amb reports that case (the same for every lane of the group),
+ * and the caller recomputes the product in full. */
+template <int K, int L, bool kShort>
+__device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
+                                           const uint64_t (&cl)[mul_digits_per_lane(K)],
+                                           const uint64_t (&ch)[mul_digits_per_lane(K)], int mode, uint32_t ea,
+                                           uint32_t eb, uint32_t &re, uint32_t (&R)[K], bool &amb) {
+  constexpr int n = K * L;
+  constexpr int KD = mul_digits_per_lane(K);
+  constexpr int ND = KD * L;               /* digits per operand */
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+
   /* normalise: chain of 2L chunks, low chunks 0..L-1 then high chunks 0..L-1 */
   uint32_t dl[KD], dh[KD];
   {
-    uint64_t cl[KD], ch[KD];
-    const uint64_t *lrow = low + g.gl * KD;     /* one base address, constant offsets */
-#pragma unroll
-    for (int i = 0; i < KD; i++) { cl[i] = lrow[i]; ch[i] = W[i]; }
-    wsync();   /* every lane has read its low columns before the digits overwrite them */
+    wsync();   /* every lane has read its columns before the digits overwrite them */
     uint64_t co_l = normalize_columns<KD>(cl, dl, 0);
     uint64_t co_h = normalize_columns<KD>(ch, dh, 0);
     /* multi-bit carries from the preceding chunk */
@@ -715,8 +747,38 @@ __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int g
     rw = digits_bits(lds, base0 - 32);
   }
   const int base = full ? p : p - 1;
-  uint32_t loc;
-  {
+  uint32_t sticky;
+  if constexpr (kShort) {
+    /* Q = A*B - E with 0 <= E < 2^(27T+36) <= 2^(p-97).  If bits [p-97, p-33) of Q hold
+     * both a zero and a one, Q + E has the same bits from p-33 up (no carry reaches them)
+     * and a nonzero part below, so mantissa, round word and sticky = 1 are those of A*B
+     * in either normalisation (round word at p-33 or p-32).  The window lies in the low
+     * half's digits WD0..WD1, in at most two lanes. */
+    constexpr int WB = p - 97, WE = p - 33;
+    constexpr int WD0 = WB / MUL_RADIX, WD1 = (WE - 1) / MUL_RADIX;
+    constexpr int WL0 = WD0 / KD, WL1 = WD1 / KD;
+    static_assert(WB >= 0 && WL1 - WL0 <= 1 && WL1 < L, "check window inside the low half, two lanes at most");
+    bool z = true, o = true;
+    auto window_part = [&](int wl) {
+      bool zz = true, oo = true;
+#pragma unroll
+      for (int k = 0; k < KD; k++) {
+        const int d0 = MUL_RADIX * (wl * KD + k);
+        const int lo = WB - d0 > 0 ? WB - d0 : 0, hi = WE - d0 < MUL_RADIX ? WE - d0 : MUL_RADIX;
+        if (hi > lo) {
+          const uint32_t m = ((1u << (hi - lo)) - 1u) << lo;
+          zz &= (dl[k] & m) == 0u;
+          oo &= (dl[k] & m) == m;
+        }
+      }
+      if (g.gl == wl) { z = zz; o = oo; }
+    };
+    window_part(WL0);
+    if constexpr (WL1 != WL0) window_part(WL1);
+    amb = g.ballot(!z) == 0 || g.ballot(!o) == 0;
+    sticky = 1u;
+  } else {
+    uint32_t loc;
     /* sticky covers bits [0, lim), lim = base - 32 in {p - 33, p - 32}: low digits below
      * digit D0 = (p - 33) / 27 are whole, digit D0 holds lim (p - 32 - 27*D0 <= 27), and
      * D0 sits in lane D0 / KD at index D0 % KD */
@@ -730,8 +792,9 @@ __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int g
     }
     const uint32_t part = dl[i_b] & ((1u << (lim - MUL_RADIX * D0)) - 1u);
     loc = g.gl < lane_b ? (pre | rest) : g.gl == lane_b ? (pre | part) : 0u;
+    sticky = g.any(loc != 0) ? 1u : 0u;
+    amb = false;
   }
-  const uint32_t sticky = g.any(loc != 0) ? 1u : 0u;
   wsync();
 
   const uint32_t sign = (ea ^ eb) & 1u;
@@ -739,7 +802,189 @@ __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int g
   if (round_mantissa<K, L>(g, H, mode, sign, rw, sticky)) E += 1;
   re = pack_exp(E, sign, 5, 0x7FFFFFFFll);
   chunk_select(R, true, H, H);
-  if (kLockstep && act != FPA_A_MUL) {
+}
+
+/*
+ * Short product.  The rounded product needs A*B only down to the round word and a
+ * sticky bit; MPFR's method (mpfr_mulhigh_n, then mpfr_round_p) applies: sum only the
+ * digit products of columns >= T, which leaves an error below 2^(27T+36) (at most 511
+ * products per column, each below 2^54), round from that sum when its bits just below the
+ * round word are not all equal, and form the full product only when they are.
+ *
+ * With chunks of KD digits, chunk pairs (alpha, beta) with alpha + beta >= L-1 hold every
+ * product of columns >= KD*(L-1); on the guard anti-diagonal alpha + beta = L-2 only the
+ * corner x + y >= G of each pair's KD x KD digit block reaches column T = KD*(L-2) + G.
+ * That is L(L+1)/2 pairs plus L-1 small corners instead of L^2 pairs.  The pairs are not
+ * spread evenly over the rounds of the product-scanning form (round r needs only lanes
+ * gl >= L-1-r), so every lane here runs its own list (fpa_mul_sched.h): two segments of a
+ * fixed b chunk and a rising a row, the window of 2*KD column sums sliding one chunk per
+ * round by register renaming, each finished chunk added into a column array in LDS
+ * (64-bit LDS atomic adds: several lanes finish the same chunk).  Only chunks >= L-1 are
+ * kept; a pair's part below is dropped (all of it lies in columns below T).
+ */
+template <int K, int L>
+struct MulShort {
+  static constexpr int n = K * L, p = 32 * n;
+  static constexpr int KD = mul_digits_per_lane(K), KDP = mul_digits_padded(K), ND = KD * L;
+  /* the check window [p-97, p-33) must lie above the error: 27T + 36 <= p - 97 */
+  static constexpr int TMAX = (p - 133) / MUL_RADIX;
+  /* corner threshold; 2KD-1 (no corner) once T clears the guard pairs' top column */
+  static constexpr int G = TMAX - KD * (L - 2) < 2 * KD - 1 ? TMAX - KD * (L - 2) : 2 * KD - 1;
+  static constexpr int T = KD * (L - 2) + G;
+  static constexpr bool kCorner = G <= 2 * KD - 2;
+  /* a's digit chunks, then the 64-bit columns of chunks L-1 .. 2L-1 */
+  static constexpr int COL0 = L * KDP;
+  static constexpr int NCOL = (L + 1) * KD;
+  static constexpr int words = COL0 + 2 * NCOL;
+  /* T >= KD*(L-1): the chunks below L-1 that are dropped hold only columns below T */
+  static constexpr bool ok = MulSched<L>::ok && G >= KD && ND < 512 && 2 * L <= 64;
+};
+
+template <int K, int L>
+__host__ __device__ constexpr bool mul_short_ok() { return MulShort<K, L>::ok; }
+
+/* LDS words per value of op_mul: the short product's, and core_mul's for its fallback */
+template <int K, int L>
+__host__ __device__ constexpr int mul_op_lds_words() {
+  return mul_short_ok<K, L>() && MulShort<K, L>::words > mul_lds_words(K, L) ? MulShort<K, L>::words
+                                                                             : mul_lds_words(K, L);
+}
+
+/* one chunk (KD columns) of the window into the column array if the chunk is kept */
+template <int L, int KD>
+__device__ __forceinline__ void mul_emit(uint64_t *cols, int c, const uint64_t *w) {
+  if (c >= L - 1) {
+    /* an opaque offset keeps the base in one register and the columns as the
+     * instructions' immediate offsets (folded, the chunk's negative start costs an add per
+     * column) */
+    int off = (c - (L - 1)) * KD;
+    asm volatile("" : "+v"(off));
+    uint64_t *dst = cols + off;
+#pragma unroll
+    for (int i = 0; i < KD; i++) __hip_atomic_fetch_add(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+}
+
+/* One segment: b chunk beta (from its group lane), a rows alpha0 .. alpha0+RS-1 from LDS */
+template <int RS, int L, int KD, int KDP>
+__device__ __forceinline__ void mul_short_segment(const Group<L> &g, const uint32_t *adig, uint64_t *cols,
+                                                  const uint32_t (&B)[KD], int alpha0, int beta) {
+  uint32_t Bc[KD];
+#pragma unroll
+  for (int j = 0; j < KD; j++) Bc[j] = g.bcast(B[j], beta);
+  uint64_t W[2 * KD];
+#pragma unroll
+  for (int i = 0; i < 2 * KD; i++) W[i] = 0;
+  /* rows start on 16-byte boundaries (KDP is a multiple of 4 words) */
+  const uint32_t *arow = static_cast<const uint32_t *>(__builtin_assume_aligned(adig + alpha0 * KDP, 16));
+  /* straight-line rounds, so the window's slide is a register renaming */
+  repeat<RS>([&](int u) {
+    uint32_t ar[KDP];
+#pragma unroll
+    for (int q = 0; q < KDP / 4; q++) {
+      const uint4 t = reinterpret_cast<const uint4 *>(arow + u * KDP)[q];
+      ar[4 * q] = t.x; ar[4 * q + 1] = t.y; ar[4 * q + 2] = t.z; ar[4 * q + 3] = t.w;
+    }
+    /* keep the padding words' loads alive: the row then stays whole 16-byte reads instead
+     * of being split into 4-byte pairs around the dead words */
+    if constexpr (KD < KDP) {
+#pragma unroll
+      for (int i = KD; i < KDP; i++) asm volatile("" ::"v"(ar[i]));
+    }
+#pragma unroll
+    for (int i = 0; i < KD; i++) {
+#pragma unroll
+      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ar[i] * Bc[j];
+    }
+    const int c = alpha0 + u + beta;
+    mul_emit<L, KD>(cols, c, W);
+    if (u == RS - 1) {
+      mul_emit<L, KD>(cols, c + 1, W + KD);
+    } else {
+#pragma unroll
+      for (int i = 0; i < KD; i++) { W[i] = W[KD + i]; W[KD + i] = 0; }
+    }
+  });
+}
+
+/* (re, R) = a * b through the short product; every group of the wave calls it together.
+ * lds: max(MulShort::words, mul_lds_words) words.  Falls back to core_mul for the whole
+ * wave when some group's rounding is not decided by the short product. */
+template <int K, int L>
+__device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds, int group_stride, int mode,
+                                               uint32_t ea, const uint32_t (&Aw)[K], uint32_t eb,
+                                               const uint32_t (&Bw)[K], uint32_t &re, uint32_t (&R)[K]) {
+  using S = MulShort<K, L>;
+  constexpr int KD = S::KD, KDP = S::KDP;
+  static_assert(S::ok, "shape without a short product");
+  static_assert(K * L + (MUL_RADIX * KD * L + 31) / 32 + 1 - K * L <= S::COL0,
+                "the recode's staging area lies below the column array");
+  const uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  uint64_t *cols = reinterpret_cast<uint64_t *>(lds + S::COL0);
+  {
+    /* clear the column array, 16 bytes per store dealt over the group's lanes */
+    constexpr int NV = (2 * S::NCOL + 3) / 4;
+    const uint4 z = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int q = 0; q < (NV + L - 1) / L; q++) {
+      const int idx = q * L + g.gl;
+      if (idx < NV) reinterpret_cast<uint4 *>(lds + S::COL0)[idx] = z;
+    }
+  }
+  uint32_t A[KD], B[KD];
+  recode_digits<K, L, KD>(g, lds, Aw, A);
+  recode_digits<K, L, KD>(g, lds, Bw, B);
+#pragma unroll
+  for (int k = 0; k < KD; k++) lds[g.gl * KDP + k] = A[k];
+  wsync();
+
+  const uint32_t sc = MulSched<L>::lane(g.gl);
+  mul_short_segment<MulSched<L>::R1, L, KD, KDP>(g, lds, cols, B, (int)(sc & 31u), (int)((sc >> 5) & 31u));
+  mul_short_segment<MulSched<L>::R2, L, KD, KDP>(g, lds, cols, B, (int)((sc >> 10) & 31u), (int)((sc >> 15) & 31u));
+  if constexpr (S::kCorner) {
+    /* the corner x + y >= G of guard pair (ca, cb): columns KD*(L-2) + x + y, i.e. column
+     * array entries x + y - KD of chunk L-1 */
+    constexpr int X0 = S::G - (KD - 1), NC = KD - X0;
+    const int ca = (int)((sc >> 20) & 31u), cb = (int)((sc >> 25) & 31u);
+    uint32_t bc[NC];
+#pragma unroll
+    for (int y = 0; y < NC; y++) bc[y] = g.bcast(B[X0 + y], cb);   /* every lane takes part */
+    if ((sc >> 30) & 1u) {
+      const uint32_t *arow = lds + ca * KDP;
+      uint64_t cs[2 * NC - 1];
+#pragma unroll
+      for (int t = 0; t < 2 * NC - 1; t++) cs[t] = 0;
+#pragma unroll
+      for (int x = 0; x < NC; x++) {
+        const uint32_t ax = arow[X0 + x];
+#pragma unroll
+        for (int y = 0; y < NC; y++)
+          if (x + y + 2 * X0 >= S::G) cs[x + y] += (uint64_t)ax * bc[y];
+      }
+#pragma unroll
+      for (int t = 0; t < 2 * NC - 1; t++)
+        if (t + 2 * X0 >= S::G)
+          __hip_atomic_fetch_add(cols + (t + 2 * X0 - KD), cs[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  }
+  wsync();
+
+  /* lane gl: low chunk gl (kept only for gl = L-1), high chunk gl = chunk L+gl */
+  uint64_t cl[KD], ch[KD];
+  {
+    const uint64_t *hrow = cols + (g.gl + 1) * KD;
+#pragma unroll
+    for (int i = 0; i < KD; i++) {
+      ch[i] = hrow[i];
+      cl[i] = g.gl == L - 1 ? cols[i] : 0ull;
+    }
+  }
+  bool amb;
+  mul_finish<K, L, true>(g, lds, cl, ch, mode, ea, eb, re, R, amb);
+  if (__builtin_expect(__ballot(amb && act == FPA_A_MUL) != 0, 0)) {
+    /* some group of the wave is undecided: the whole wave forms its products in full */
+    core_mul<K, L, true>(g, lds, group_stride, mode, ea, Aw, eb, Bw, re, R);
+  } else if (act != FPA_A_MUL) {
     re = act;
 #pragma unroll
     for (int i = 0; i < K; i++) R[i] = 0;
@@ -751,7 +996,10 @@ __device__ __forceinline__ void op_mul(const fpa_arguments &a, const Slot &s, co
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
-  core_mul<K, L, true>(g, lds, group_stride, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  if constexpr (mul_short_ok<K, L>())
+    core_mul_short<K, L>(g, lds, group_stride, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  else
+    core_mul<K, L, true>(g, lds, group_stride, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
   store_result(a, s, g.gl, re, R);
 }
 

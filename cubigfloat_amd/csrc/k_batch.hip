@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 template <int K, int L>
 __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
   constexpr int VPW = 64 / L;
-  constexpr int SW = cmax(ScratchWords<3, K, L>::value, ScratchWords<1, K, L>::value);
+  /* core_mul's scratch (not op_mul's, which also holds the short product's columns) */
+  constexpr int SW = cmax(((mul_lds_words(K, L) + 3) / 4) * 4 + 4, ScratchWords<1, K, L>::value);
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
   Group<L> g;
   const int wib = threadIdx.x >> 6;
