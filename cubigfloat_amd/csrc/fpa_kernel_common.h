@@ -38,15 +38,21 @@ struct ScratchWords {
   static constexpr int value = ((raw + 3) / 4) * 4 + 4;
 };
 
+/* Waves per block: 4, or 1 for mul with a short product, whose LDS then packs by the
+ * wave (10 waves per CU at 4096 bits instead of 2 blocks of 4). */
+template <int OP, int K, int L>
+__host__ __device__ constexpr int fpa_waves_per_block() { return OP == 3 && mul_short_ok<K, L>() ? 1 : 4; }
+
 template <int K, int L, int OP>
 __global__ FPA_KERNEL_ATTRS(OP, K, L) void fpa_kernel(fpa_arguments a) {
   constexpr int VPW = 64 / L;
+  constexpr int WPB = fpa_waves_per_block<OP, K, L>();
   constexpr int SW = ScratchWords<OP, K, L>::value;
-  __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
+  __shared__ __attribute__((aligned(16))) uint32_t smem[WPB * VPW * SW];
   Group<L> g;
   const int wib = threadIdx.x >> 6;
   const int gid = g.lane / L;
-  const uint64_t vv = ((uint64_t)blockIdx.x * 4 + wib) * VPW + gid;
+  const uint64_t vv = ((uint64_t)blockIdx.x * WPB + wib) * VPW + gid;
   Slot s;
   s.valid = vv < a.count;
   s.v = s.valid ? (uint32_t)vv : a.count - 1;
@@ -63,9 +69,10 @@ template <int K, int L, int OP>
 static int launch_shape(const fpa_arguments &a, hipStream_t st) {
   constexpr int VPW = 64 / L;
   if (a.count == 0) return 0;
+  constexpr int WPB = fpa_waves_per_block<OP, K, L>();
   uint64_t waves = ((uint64_t)a.count + VPW - 1) / VPW;
-  uint64_t blocks = (waves + 3) / 4;
-  hipLaunchKernelGGL((fpa_kernel<K, L, OP>), dim3((uint32_t)blocks), dim3(256), 0, st, a);
+  uint64_t blocks = (waves + WPB - 1) / WPB;
+  hipLaunchKernelGGL((fpa_kernel<K, L, OP>), dim3((uint32_t)blocks), dim3(64 * WPB), 0, st, a);
   return (int)hipGetLastError();
 }
 

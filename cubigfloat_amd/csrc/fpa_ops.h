@@ -819,8 +819,8 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
  * gl >= L-1-r), so every lane here runs its own list (fpa_mul_sched.h): two segments of a
  * fixed b chunk and a rising a row, the window of 2*KD column sums sliding one chunk per
  * round by register renaming, each finished chunk added into a column array in LDS
- * (64-bit LDS atomic adds: several lanes finish the same chunk).  Only chunks >= L-1 are
- * kept; a pair's part below is dropped (all of it lies in columns below T).
+ * (64-bit LDS atomic adds: several lanes finish the same chunk).  Only the columns >= T are
+ * kept (chunk L-1 from column T on, and the chunks above it); what falls below is dropped.
  */
 template <int K, int L>
 struct MulShort {
@@ -850,8 +850,9 @@ __host__ __device__ constexpr int mul_op_lds_words() {
                                                                              : mul_lds_words(K, L);
 }
 
-/* one chunk (KD columns) of the window into the column array if the chunk is kept */
-template <int L, int KD>
+/* one chunk (KD columns) of the window into the column array if the chunk is kept; of
+ * chunk L-1 only columns C0.. (those at or above T) */
+template <int L, int KD, int C0>
 __device__ __forceinline__ void mul_emit(uint64_t *cols, int c, const uint64_t *w) {
   if (c >= L - 1) {
     /* an opaque offset keeps the base in one register and the columns as the
@@ -860,49 +861,76 @@ __device__ __forceinline__ void mul_emit(uint64_t *cols, int c, const uint64_t *
     int off = (c - (L - 1)) * KD;
     asm volatile("" : "+v"(off));
     uint64_t *dst = cols + off;
+    if (c >= L) {
 #pragma unroll
-    for (int i = 0; i < KD; i++) __hip_atomic_fetch_add(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      for (int i = 0; i < C0; i++) __hip_atomic_fetch_add(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+#pragma unroll
+    for (int i = C0; i < KD; i++) __hip_atomic_fetch_add(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
 }
 
-/* One segment: b chunk beta (from its group lane), a rows alpha0 .. alpha0+RS-1 from LDS */
-template <int RS, int L, int KD, int KDP>
-__device__ __forceinline__ void mul_short_segment(const Group<L> &g, const uint32_t *adig, uint64_t *cols,
-                                                  const uint32_t (&B)[KD], int alpha0, int beta) {
-  uint32_t Bc[KD];
+/* a's digit row alpha (KDP words from 16-byte-aligned LDS) */
+template <int KD, int KDP>
+__device__ __forceinline__ void mul_load_row(const uint32_t *adig, int alpha, uint32_t (&ar)[KDP]) {
+  const uint32_t *row = static_cast<const uint32_t *>(__builtin_assume_aligned(adig + alpha * KDP, 16));
 #pragma unroll
-  for (int j = 0; j < KD; j++) Bc[j] = g.bcast(B[j], beta);
+  for (int q = 0; q < KDP / 4; q++) {
+    const uint4 t = reinterpret_cast<const uint4 *>(row)[q];
+    ar[4 * q] = t.x; ar[4 * q + 1] = t.y; ar[4 * q + 2] = t.z; ar[4 * q + 3] = t.w;
+  }
+  /* keep the padding words' loads alive: the row then stays whole 16-byte reads instead
+   * of being split into 4-byte pairs around the dead words */
+  if constexpr (KD < KDP) {
+#pragma unroll
+    for (int i = KD; i < KDP; i++) asm volatile("" ::"v"(ar[i]));
+  }
+}
+
+/* The lane's two segments: R1 rounds of b chunk beta1 with a rows alpha1.., then R2 rounds
+ * of beta2 with rows alpha2...  Both b chunks are fetched up front and every round's a row
+ * is read before the previous round's atomics are issued (which the compiler may not move
+ * loads above), so the LDS latency overlaps a round of multiply-adds; chunk L-1 keeps its
+ * columns C0.. */
+template <int R1, int R2, int L, int KD, int KDP, int C0>
+__device__ __forceinline__ void mul_short_rounds(const Group<L> &g, const uint32_t *adig, uint64_t *cols,
+                                                 const uint32_t (&B)[KD], int alpha1, int beta1, int alpha2,
+                                                 int beta2) {
+  constexpr int R = R1 + R2;
+  uint32_t Bc1[KD], Bc2[KD];
+#pragma unroll
+  for (int j = 0; j < KD; j++) {
+    Bc1[j] = g.bcast(B[j], beta1);
+    Bc2[j] = g.bcast(B[j], beta2);
+  }
   uint64_t W[2 * KD];
 #pragma unroll
   for (int i = 0; i < 2 * KD; i++) W[i] = 0;
-  /* rows start on 16-byte boundaries (KDP is a multiple of 4 words) */
-  const uint32_t *arow = static_cast<const uint32_t *>(__builtin_assume_aligned(adig + alpha0 * KDP, 16));
+  uint32_t ar[KDP];
+  mul_load_row<KD, KDP>(adig, alpha1, ar);
   /* straight-line rounds, so the window's slide is a register renaming */
-  repeat<RS>([&](int u) {
-    uint32_t ar[KDP];
-#pragma unroll
-    for (int q = 0; q < KDP / 4; q++) {
-      const uint4 t = reinterpret_cast<const uint4 *>(arow + u * KDP)[q];
-      ar[4 * q] = t.x; ar[4 * q + 1] = t.y; ar[4 * q + 2] = t.z; ar[4 * q + 3] = t.w;
-    }
-    /* keep the padding words' loads alive: the row then stays whole 16-byte reads instead
-     * of being split into 4-byte pairs around the dead words */
-    if constexpr (KD < KDP) {
-#pragma unroll
-      for (int i = KD; i < KDP; i++) asm volatile("" ::"v"(ar[i]));
-    }
+  repeat<R>([&](int t) {
+    const bool seg1 = t < R1;
+    uint32_t nx[KDP];
+    if (t + 1 < R) mul_load_row<KD, KDP>(adig, t + 1 < R1 ? alpha1 + t + 1 : alpha2 + (t + 1 - R1), nx);
 #pragma unroll
     for (int i = 0; i < KD; i++) {
 #pragma unroll
-      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ar[i] * Bc[j];
+      for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ar[i] * (seg1 ? Bc1[j] : Bc2[j]);
     }
-    const int c = alpha0 + u + beta;
-    mul_emit<L, KD>(cols, c, W);
-    if (u == RS - 1) {
-      mul_emit<L, KD>(cols, c + 1, W + KD);
+    const int c = seg1 ? alpha1 + t + beta1 : alpha2 + (t - R1) + beta2;
+    mul_emit<L, KD, C0>(cols, c, W);
+    if (t == R1 - 1 || t == R - 1) {
+      mul_emit<L, KD, C0>(cols, c + 1, W + KD);
+#pragma unroll
+      for (int i = 0; i < 2 * KD; i++) W[i] = 0;
     } else {
 #pragma unroll
       for (int i = 0; i < KD; i++) { W[i] = W[KD + i]; W[KD + i] = 0; }
+    }
+    if (t + 1 < R) {
+#pragma unroll
+      for (int i = 0; i < KDP; i++) ar[i] = nx[i];
     }
   });
 }
@@ -939,8 +967,9 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
   wsync();
 
   const uint32_t sc = MulSched<L>::lane(g.gl);
-  mul_short_segment<MulSched<L>::R1, L, KD, KDP>(g, lds, cols, B, (int)(sc & 31u), (int)((sc >> 5) & 31u));
-  mul_short_segment<MulSched<L>::R2, L, KD, KDP>(g, lds, cols, B, (int)((sc >> 10) & 31u), (int)((sc >> 15) & 31u));
+  constexpr int C0 = S::T - KD * (L - 1);
+  mul_short_rounds<MulSched<L>::R1, MulSched<L>::R2, L, KD, KDP, C0>(
+      g, lds, cols, B, (int)(sc & 31u), (int)((sc >> 5) & 31u), (int)((sc >> 10) & 31u), (int)((sc >> 15) & 31u));
   if constexpr (S::kCorner) {
     /* the corner x + y >= G of guard pair (ca, cb): columns KD*(L-2) + x + y, i.e. column
      * array entries x + y - KD of chunk L-1 */

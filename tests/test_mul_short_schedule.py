@@ -3,7 +3,7 @@
 For every compiled mul shape with a short product, the lane schedules of
 cubigfloat_amd/csrc/fpa_mul_sched.h are replayed on Python integers exactly as the kernel
 sums them: each lane's two segments of chunk pairs (window of 2*KD columns, chunks below
-L-1 dropped) plus the guard corners.  Pinned here:
+L-1 dropped) plus the guard corners.  Columns below T are dropped, as in the kernel.  Pinned here:
   * the schedule covers every chunk pair with alpha + beta >= L-1 exactly once, and no pair
     of the guard anti-diagonal twice;
   * Q <= A*B < Q + 2^(27T+36) for random, all-ones and alphabet operands (the error bound
@@ -73,10 +73,13 @@ def short_q(A, B, K, L):
     a, b = digits(A, ND), digits(B, ND)
     cols = {}
 
+    T = P["T"]
+
     def emit(c, w):
         if c >= L - 1:
             for i in range(KD):
-                cols[c * KD + i] = cols.get(c * KD + i, 0) + w[i]
+                if c * KD + i >= T:       # chunk L-1 keeps its columns from T on
+                    cols[c * KD + i] = cols.get(c * KD + i, 0) + w[i]
 
     for sc in words:
         for (a0, b0, rs) in ((sc & 31, (sc >> 5) & 31, R1), ((sc >> 10) & 31, (sc >> 15) & 31, R2)):
