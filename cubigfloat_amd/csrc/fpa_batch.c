@@ -32,6 +32,7 @@
 int fpa_impl_dot_launch(const fpa_dot_arguments *d, int K, int L, void *stream);
 int fpa_impl_persist_kind(int op, uint32_t precision, int *K, int *L);
 int fpa_impl_persist_blocks_per_cu(void);
+int fpa_impl_mul_rounds(int K, int L);
 int fpa_impl_persist_launch(const fpa_persist_arguments *P, uint32_t grid, void *stream);
 void *fpa_get_stream(void);
 
@@ -70,11 +71,16 @@ int fpa_batch_precision_supported(uint32_t precision) { return fpa_precision_sup
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-/* relative time one wavefront spends on its 64/L values of a segment */
+/* relative time one wavefront spends on its 64/L values of a segment (mul: product
+ * rounds of KD x KD multiply-adds per lane, fewer than L where the shape forms a short
+ * product) */
 static double wave_cost(int op, int K, int L) {
   int n = K * L, KD = (32 * K + 26) / 27;
   switch (op) {
-    case FPA_OP_MUL: return (double)L * KD * KD + 40.0 * KD;
+    case FPA_OP_MUL: {
+      const int rounds = fpa_impl_mul_rounds(K, L);
+      return (double)(rounds > 0 ? rounds : L) * KD * KD + 40.0 * KD;
+    }
     case FPA_OP_DIV: return (double)(n + 1) * (3 * K + 40);
     case FPA_OP_SQRT: return (double)n * (3 * K + 50);
     default: return 8.0 * K + 60.0;

@@ -843,6 +843,13 @@ struct MulShort {
 template <int K, int L>
 __host__ __device__ constexpr bool mul_short_ok() { return MulShort<K, L>::ok; }
 
+/* product rounds per lane: the short product's schedule, or L for the full scan */
+template <int K, int L>
+__host__ __device__ constexpr int mul_rounds() {
+  if constexpr (mul_short_ok<K, L>()) return MulSched<L>::R1 + MulSched<L>::R2;
+  else return L;
+}
+
 /* LDS words per value of op_mul: the short product's, and core_mul's for its fallback */
 template <int K, int L>
 __host__ __device__ constexpr int mul_op_lds_words() {

@@ -101,7 +101,7 @@ __device__ __forceinline__ uint32_t run_kind(const fpa_persist_arguments &P, uin
 /* The segment table travels in the kernel arguments: pointers read from there are known to
  * be global, so every operand access is a global instruction (read from a table in memory
  * they would be generic pointers and FLAT accesses). */
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fpa_batch_persistent_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void fpa_batch_persistent_kernel(
     fpa_persist_arguments P) {
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * kPersistWaveLds];
   const int wib = threadIdx.x >> 6;
