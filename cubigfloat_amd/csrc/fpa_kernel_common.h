@@ -33,6 +33,7 @@ struct ScratchWords {
                            : OP <= 2 ? n + 1
                            : OP == 3 ? mul_op_lds_words<K, L>()
                            : OP == 4 ? div_lds_words(K, L)
+                           : sqrt_lazy_shape(K) && sqrt_lazy_lds_words<K, L>() > n + 1 ? sqrt_lazy_lds_words<K, L>()
                            : n + 1;
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
   static constexpr int value = ((raw + 3) / 4) * 4 + 4;

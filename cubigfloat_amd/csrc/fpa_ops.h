@@ -14,9 +14,11 @@
  *   div      fpa_warp_div.h:127-201 -> op_div: long division in radix 2^27 with the
  *            remainder kept as lazy signed 64-bit column sums (one multiply-add per column
  *            and digit), FP64 digit estimates, redundant digits, exact end correction.
- *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root with 32-bit
- *            digits in a fixed frame: R <- R*2^32 - d*(2S + d*2^(32 pos)), the digit
- *            estimated from (R/2)/S's top words, add-back while negative.
+ *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root; up to 24
+ *            limbs per lane in radix 2^27 on div's lazy column frame (core_sqrt_lazy),
+ *            with 32 limbs per lane with 32-bit digits in a fixed frame:
+ *            R <- R*2^32 - d*(2S + d*2^(32 pos)), the digit estimated from (R/2)/S's top
+ *            words, add-back while negative (core_sqrt).
  * Rounding uses fpa_round's decision (fpa_scalar.h) on the exact (round word, sticky)
  * pair; a carry out of the rounding increment becomes 0x80..0 one binade up.
  */
@@ -1119,6 +1121,13 @@ __host__ __device__ constexpr int div_lds_words(int K, int L) {
                                                                    : div_steps(K * L) + 3;
 }
 
+/* LDS words per value of the lazy sqrt: recoding stage or the root digits (+ three zeros) */
+template <int K, int L>
+__host__ __device__ constexpr int sqrt_lazy_lds_words() {
+  return div_pad_words(K, L) + K * L + 2 > div_digits_per_lane(K) * L + 3 ? div_pad_words(K, L) + K * L + 2
+                                                                          : div_digits_per_lane(K) * L + 3;
+}
+
 /* x as held by group lane src, 64 bits */
 template <int L>
 __device__ __forceinline__ int64_t bcast64(const Group<L> &g, int64_t x, int src) {
@@ -1132,11 +1141,11 @@ __device__ __forceinline__ double i64_to_f64(int64_t x) {
 }
 
 /* Digits of X (n = K*L words) in radix 2^27 aligned to the top of KD*L digits: digit k of
- * the value = bits [27k - PAD, 27k - PAD + 27) of X, PAD = 27*KD*L - 32n (bits below X's bit
+ * the value = bits [27k - PAD + extra, +27) of X, PAD = 27*KD*L - 32n (bits below X's bit
  * 0 read as zero).  lds: ZW + n + 2 words, ZW = ceil(PAD / 32). */
 template <int K, int L, int KD>
 __device__ __forceinline__ void recode_digits_top(const Group<L> &g, uint32_t *lds, const uint32_t (&X)[K],
-                                                  uint32_t (&D)[KD]) {
+                                                  uint32_t (&D)[KD], int extra = 0) {
   constexpr int n = K * L, PAD = DIV_RADIX * KD * L - 32 * n, ZW = (PAD + 31) / 32;
   constexpr uint32_t M = (1u << DIV_RADIX) - 1;
   static_assert(PAD >= 0, "digits cover the value");
@@ -1148,9 +1157,10 @@ __device__ __forceinline__ void recode_digits_top(const Group<L> &g, uint32_t *l
   if (g.gl == L - 1) { lds[ZW + n] = 0; lds[ZW + n + 1] = 0; }
   wsync();
   /* the lane's digits are bits [b, b + 27*KD) of the staged string (its bit 32*ZW is X's
-   * bit 0); the highest word read is floor(b/32) + NW <= ZW + n + 1 */
+   * bit 0); the highest word read is floor(b/32) + NW <= ZW + n + 1.  extra = 1 takes the
+   * digits of X / 2 (one bit less padding; extra <= PAD) */
   constexpr int NW = (DIV_RADIX * KD + 31) / 32;
-  const int b = DIV_RADIX * KD * g.gl - PAD + 32 * ZW;
+  const int b = DIV_RADIX * KD * g.gl - PAD + extra + 32 * ZW;
   const uint32_t *src = lds + (b >> 5);
   const uint32_t sh = (uint32_t)b & 31u;
   uint32_t win[NW + 1];
@@ -1659,12 +1669,226 @@ __device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int 
   chunk_select(Sr, true, S, S);
 }
 
+/* The lazy-column root (below) for shapes with up to 24 limbs per lane; with 32 its
+ * 38 64-bit columns and their digits spill, and the 32-bit frame root above is kept. */
+__host__ __device__ constexpr bool sqrt_lazy_shape(int K) { return K <= 24; }
+template <int K, int L>
+__device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
+                                               const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]);
+
 template <int K, int L>
 __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
-  core_sqrt<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  if constexpr (sqrt_lazy_shape(K))
+    core_sqrt_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  else
+    core_sqrt<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
   store_result(a, s, g.gl, re, R);
+}
+
+/* ------------------- sqrt with lazy radix-2^27 column sums ------------------- */
+/*
+ * Digit-by-digit square root in radix beta = 2^27 on div's lazy frame.  With F = KD*L root
+ * digits and t = 27F - p >= 1 extra root bits, the radicand is M = A * 2^(54F - p - o)
+ * (o = 1 for an odd exponent: the radicand is A/2) and Z = floor(sqrt(M)) has exactly 27F
+ * bits: the mantissa is Z >> t, the round word the 32 bits below it, the sticky bit the
+ * rest of Z and the remainder M - Z^2.  M's low F digits are zero, so the frame starts as
+ * U = M / beta^F (A's digits, top-aligned) and only zeros enter it from below.  T = 2S sits
+ * in the same F slots (root digit k at slot pos = F-1-k, kept negated in nT).  Step k:
+ *   1. U <- U * beta: columns move up one slot, the top one folds into the new top column;
+ *   2. the digit: k = 0: q = isqrt of the top column (exact, below 2^54); k = 1: T's second
+ *      digit is not there yet and the q^2 term is 2^-27 of q*T, so q solves
+ *      q^2 + 2 q0 beta q = V as V / (q0 beta + sqrt((q0 beta)^2 + V)) in FP64 (nothing
+ *      cancels); k >= 2: q = V / (T_{F-1} beta + T_{F-2}) as in div (the q^2 term and T's
+ *      lower digits are below 2^-53 of it), V from the top two columns;
+ *   3. U -= q * T (one v_mad_i64_i32 per column), the owner of slot pos subtracts q^2 and
+ *      sets T's slot to 2q.
+ * The digits are redundant as in div and |q * 2q_j| < 2^57, so one carry level every 64
+ * steps keeps the columns below 2^63.  At the end the estimate of R/T decides S unless it
+ * lies within 2^-16 of 0 or 1; then R and T are normalised exactly and S moves by one
+ * (R -> R + T - 1 or R - T - 1).  The root digits (-nT/2) go to LDS, are carry-normalised
+ * only when one lies outside [0, beta), and the mantissa, round word and sticky bit are
+ * read from them.  The reference's root (fpa_warp_sqrt.h:195-254) runs 32-bit digits
+ * with exact remainders instead.
+ */
+template <int K, int L>
+__device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
+                                               const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]) {
+  constexpr int n = K * L, p = 32 * n;
+  constexpr int KD = div_digits_per_lane(K), F = KD * L, TX = DIV_RADIX * F - p;
+  constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
+  constexpr double BETA_D = 134217728.0;   /* 2^27 */
+  static_assert(KD >= 2, "the top lane holds the two top columns");
+  static_assert(TX >= 1, "at least one root bit below the mantissa");
+  const uint32_t act = k_sqrt_tab[fpa_classify(ea)];
+  if (act != FPA_A_SQRT) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) Sr[i] = 0;
+    return;
+  }
+  int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
+  const int o = (int)(e & 1);
+  e += o;
+  const int64_t er = e / 2;
+  uint32_t ad[KD];
+  recode_digits_top<K, L, KD>(g, lds, A, ad, o);
+  int64_t C[KD];
+  int32_t nT[KD];
+#pragma unroll
+  for (int i = 0; i < KD; i++) { C[i] = (int64_t)ad[i]; nT[i] = 0; }
+  const bool top = g.gl == L - 1;
+  double inv32 = 0.0;                      /* 2^32 / (T_{F-1} beta + T_{F-2}), from step 2 on */
+  int32_t q0 = 0;
+
+  auto step = [&](int k, int idx, bool own, int kind) __attribute__((always_inline)) {
+    /* U <- U * beta */
+    const int64_t so = C[KD - 1];
+    const uint32_t inlo = g.from_prev_row0((uint32_t)so), inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
+#pragma unroll
+    for (int i = KD - 1; i > 0; i--) C[i] = C[i - 1];
+    C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
+    const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
+    C[KD - 1] = (int64_t)((uint64_t)C[KD - 1] + fold);
+    int32_t q;
+    if (kind == 0) {
+      q = (int32_t)fpa_isqrt64((uint64_t)C[KD - 1]);
+    } else {
+      const double Vs = __builtin_fma(i64_to_f64(C[KD - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KD - 2] >> 32));
+      if (kind == 1) {
+        const double a = (double)q0 * BETA_D, V = Vs * 0x1p32;
+        q = (int32_t)__builtin_floor(V / (a + __builtin_sqrt(__builtin_fma(a, a, V))));
+      } else {
+        q = (int32_t)__builtin_floor(Vs * inv32);
+      }
+    }
+    q = (int32_t)g.bcast_top((uint32_t)q);
+#pragma unroll
+    for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nT[i] + C[i];
+    if (own) {
+#pragma unroll
+      for (int i = 0; i < KD; i++) {
+        if (i == idx) {
+          C[i] -= (int64_t)q * q;
+          nT[i] = -2 * q;
+        }
+      }
+    }
+    if (kind == 0) q0 = q;
+    if (kind == 1) {
+      const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
+      inv32 = 0x1p32 / tt;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (F > 64) {
+      /* one carry level every 64 steps keeps the columns far below 2^63 */
+      if ((k & 63) == 63) {
+        int64_t car[KD];
+#pragma unroll
+        for (int i = 0; i < KD; i++) car[i] = C[i] >> DIV_RADIX;
+        const uint32_t clo = g.from_prev_row0((uint32_t)car[KD - 1]);
+        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KD - 1] >> 32));
+        const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
+#pragma unroll
+        for (int i = KD - 1; i >= 0; i--) {
+          const int64_t keep = (top && i == KD - 1) ? C[i] : (C[i] & (int64_t)MASK);
+          C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
+        }
+      }
+    }
+  };
+  wsync();   /* the recoding reads of lds are done before the digits overwrite them */
+  /* trips of KD steps: step k's slot pos = F-1-k is index KD-1-u of lane L-1-trip */
+  repeat<KD>([&](int u) __attribute__((always_inline)) {
+    step(u, KD - 1 - u, top, u == 0 ? 0 : u == 1 ? 1 : 2);
+  });
+  for (int tr = 1; tr < L; tr++) {
+    const bool own = g.gl == L - 1 - tr;
+    repeat<KD>([&](int u) __attribute__((always_inline)) { step(tr * KD + u, KD - 1 - u, own, 2); });
+  }
+
+  /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
+  const double r = __builtin_fma(i64_to_f64(C[KD - 1]), BETA_D, i64_to_f64(C[KD - 2])) * inv32 * 0x1p-32;
+  const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
+  uint32_t rem_nz = 1;
+  int32_t delta = 0;
+  if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
+    int64_t Tn[KD];
+#pragma unroll
+    for (int i = 0; i < KD; i++) Tn[i] = -(int64_t)nT[i];
+    const int64_t Rtop = normalize_exact<KD, L>(g, C);
+    const int64_t Ttop = normalize_exact<KD, L>(g, Tn);
+    int cmp;                               /* sign of R - T */
+    if (Rtop != Ttop) {
+      cmp = Rtop > Ttop ? 1 : -1;
+    } else {
+      int c = 0;
+#pragma unroll
+      for (int i = KD - 1; i >= 0; i--)
+        if (c == 0 && C[i] != Tn[i]) c = C[i] > Tn[i] ? 1 : -1;
+      const uint64_t diff = g.ballot(c != 0);
+      cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
+    }
+    /* S += delta: R -> R - delta * T - 1 for delta = +-1 */
+    delta = Rtop < 0 ? -1 : cmp > 0 ? 1 : 0;
+    int64_t Rhi = Rtop;
+    if (delta != 0) {
+#pragma unroll
+      for (int i = 0; i < KD; i++) C[i] -= (int64_t)delta * Tn[i];
+      if (g.gl == 0) C[0] -= 1;
+      Rhi = normalize_exact<KD, L>(g, C);
+    }
+    /* R lies in [0, 2S], which reaches past beta^F: its part above the F digits counts */
+    uint32_t nz = 0;
+#pragma unroll
+    for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
+    rem_nz = (Rhi != 0 || g.any(nz != 0)) ? 1u : 0u;
+  }
+  /* root digits S_k = -nT/2 at slot k, then three zero words */
+  int32_t *sd = reinterpret_cast<int32_t *>(lds);
+  wsync();
+#pragma unroll
+  for (int i = 0; i < KD; i++) sd[g.gl * KD + i] = -nT[i] / 2 + (g.gl == 0 && i == 0 ? delta : 0);
+  if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
+  wsync();
+  {
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < KD; i++) bad |= (uint32_t)sd[g.gl * KD + i] > MASK;
+    if (__builtin_expect(__ballot(bad) != 0, 0)) {
+      int64_t X[KD];
+#pragma unroll
+      for (int i = 0; i < KD; i++) X[i] = (int64_t)sd[g.gl * KD + i];
+      normalize_exact<KD, L>(g, X);   /* S is below beta^F: no carry out */
+      wsync();
+#pragma unroll
+      for (int i = 0; i < KD; i++) sd[g.gl * KD + i] = (int32_t)X[i];
+      wsync();
+    }
+  }
+  /* Z has 27F bits, its top bit set: mantissa = bits [TX, TX + p), round word below */
+  const uint32_t *su = reinterpret_cast<const uint32_t *>(sd);
+  uint32_t Mw[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) Mw[i] = digits_bits(su, TX + 32 * (g.gl * K + i));
+  uint32_t rw, sticky = rem_nz;
+  if constexpr (TX >= 32) {
+    rw = digits_bits(su, TX - 32);
+    /* sticky bits [0, TX - 32): whole digits below digit D0, part of digit D0 */
+    constexpr int LIM = TX - 32, D0 = LIM / DIV_RADIX, REM = LIM % DIV_RADIX;
+    uint32_t loc = 0;
+    for (int k = g.gl; k < D0; k += L) loc |= su[k];
+    if (REM > 0 && g.gl == 0) loc |= su[D0] & ((1u << REM) - 1u);
+    sticky |= g.any(loc != 0) ? 1u : 0u;
+  } else {
+    rw = digits_bits(su, 0) << (32 - TX);
+  }
+  wsync();
+  int64_t E = er + FPA_BIAS_LL;
+  if (round_mantissa<K, L>(g, Mw, mode, 0u, rw, sticky)) E += 1;
+  re = pack_exp(E, 0u, 0, 0x7FFFFFFFll);
+  chunk_select(Sr, true, Mw, Mw);
 }
 
 }  // namespace fpa
