@@ -944,10 +944,13 @@ __device__ __forceinline__ void mul_short_rounds(const Group<L> &g, const uint32
   });
 }
 
-/* (re, R) = a * b through the short product; every group of the wave calls it together.
- * lds: max(MulShort::words, mul_lds_words) words.  Falls back to core_mul for the whole
- * wave when some group's rounding is not decided by the short product. */
-template <int K, int L>
+/* (re, R) = a * b through the short product.  lds: max(MulShort::words, mul_lds_words)
+ * words.  kLockstep: every group of the wave calls it together (the element-wise kernel),
+ * and the whole wave falls back to core_mul when some group's rounding is not decided by
+ * the short product.  Without it (dot chains, whose groups run chains of different
+ * lengths) a group returns class results at once and falls back on its own; every step
+ * here reads only the group's own lanes and LDS, so that stays exact. */
+template <int K, int L, bool kLockstep = true>
 __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds, int group_stride, int mode,
                                                uint32_t ea, const uint32_t (&Aw)[K], uint32_t eb,
                                                const uint32_t (&Bw)[K], uint32_t &re, uint32_t (&R)[K]) {
@@ -957,6 +960,12 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
   static_assert(K * L + (MUL_RADIX * KD * L + 31) / 32 + 1 - K * L <= S::COL0,
                 "the recode's staging area lies below the column array");
   const uint32_t act = k_mul_tab[fpa_classify(ea) * 9 + fpa_classify(eb)];
+  if (!kLockstep && act != FPA_A_MUL) {
+    re = act;
+#pragma unroll
+    for (int i = 0; i < K; i++) R[i] = 0;
+    return;
+  }
   uint64_t *cols = reinterpret_cast<uint64_t *>(lds + S::COL0);
   {
     /* clear the column array, 16 bytes per store dealt over the group's lanes */
@@ -1019,7 +1028,10 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
   }
   bool amb;
   mul_finish<K, L, true>(g, lds, cl, ch, mode, ea, eb, re, R, amb);
-  if (__builtin_expect(__ballot(amb && act == FPA_A_MUL) != 0, 0)) {
+  if constexpr (!kLockstep) {
+    /* this group is undecided: it forms its product in full */
+    if (__builtin_expect(amb, 0)) core_mul<K, L, false>(g, lds, group_stride, mode, ea, Aw, eb, Bw, re, R);
+  } else if (__builtin_expect(__ballot(amb && act == FPA_A_MUL) != 0, 0)) {
     /* some group of the wave is undecided: the whole wave forms its products in full */
     core_mul<K, L, true>(g, lds, group_stride, mode, ea, Aw, eb, Bw, re, R);
   } else if (act != FPA_A_MUL) {

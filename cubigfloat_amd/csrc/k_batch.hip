@@ -132,8 +132,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 template <int K, int L>
 __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
   constexpr int VPW = 64 / L;
-  /* core_mul's scratch (not op_mul's, which also holds the short product's columns) */
-  constexpr int SW = cmax(((mul_lds_words(K, L) + 3) / 4) * 4 + 4, ScratchWords<1, K, L>::value);
+  /* the mul core's scratch (with the short product's columns where the shape has one) and
+   * the add core's */
+  constexpr int SW = cmax(((mul_op_lds_words<K, L>() + 3) / 4) * 4 + 4, ScratchWords<1, K, L>::value);
   __shared__ __attribute__((aligned(16))) uint32_t smem[4 * VPW * SW];
   Group<L> g;
   const int wib = threadIdx.x >> 6;
@@ -151,9 +152,14 @@ __global__ __launch_bounds__(256) void fpa_dot_kernel(fpa_dot_arguments d) {
     uint32_t X[K], Y[K], P[K], pe;
     load_chunk(X, d.x_limbs + k * n + (size_t)g.gl * K);
     load_chunk(Y, d.y_limbs + k * n + (size_t)g.gl * K);
-    /* A group of 16 lanes is one DPP row: the lockstep window shift reads no other group,
-     * so it stays exact while the wave's groups run chains of different lengths. */
-    core_mul<K, L, L == 16>(g, lds, SW, d.mode, d.x_exps[k], X, d.y_exps[k], Y, pe, P);
+    /* The short product where the shape has one (each group decides its own fallback);
+     * else the full scan: a group of 16 lanes is one DPP row, whose lockstep window shift
+     * reads no other group, so it stays exact while the wave's groups run chains of
+     * different lengths. */
+    if constexpr (mul_short_ok<K, L>())
+      core_mul_short<K, L, false>(g, lds, SW, d.mode, d.x_exps[k], X, d.y_exps[k], Y, pe, P);
+    else
+      core_mul<K, L, L == 16>(g, lds, SW, d.mode, d.x_exps[k], X, d.y_exps[k], Y, pe, P);
     uint32_t se, S[K];
     core_addsub<K, L>(g, lds, d.mode, false, acc_e, acc, pe, P, se, S);
     acc_e = se;

@@ -145,6 +145,35 @@ def test_dot_every_shape_vs_oracle(p):
         F.set_variant("dot", p, keep)
 
 
+@pytest.mark.parametrize("p", [2048, 4096, 8192])
+def test_dot_every_shape_exact_products(p):
+    """Products whose low half is zero (two-limb mantissas, an exact product) mixed with
+    random ones, so within a wave some groups cannot decide their rounding from the
+    short product and form their product in full while the others go on, on chains of
+    different lengths."""
+    rng = np.random.default_rng(p + 2)
+    lengths = rng.integers(0, 20, 90).astype(np.uint32)
+    total = int(lengths.sum())
+    n = p // 32
+    xe, xl, ye, yl = F.generate(2, 6, "mul", p, total)
+    xl, yl = xl.reshape(-1, n).copy(), yl.reshape(-1, n).copy()
+    short = rng.random(total) < 0.4
+    xl[short, :n - 2] = 0
+    yl[short, :n - 2] = 0
+    xl, yl = xl.reshape(-1), yl.reshape(-1)
+    for mode in ("rne", "ru"):
+        oe, ol = O.dot(p, lengths, xe, xl, ye, yl, mode=mode)
+        keep = F.get_variant("dot", p)
+        try:
+            for v, (k, l) in enumerate(F.variants("dot", p)):
+                F.set_variant("dot", p, v)
+                say(f"[gpu] dot exact products p={p} shape K={k} L={l} mode={mode}")
+                re, rl = F.dot_chains(p, lengths, xe, xl, ye, yl, mode=F.MODES[mode])
+                assert O.results_equal(re, rl, oe, ol, p).all(), (k, l, mode)
+        finally:
+            F.set_variant("dot", p, keep)
+
+
 def test_dot_config4_shape_sample():
     lengths, xe, xl, ye, yl = W.dot_workload(256, 2048, 11)
     say(f"[gpu] dot config-4 sample chains=256 elements={int(lengths.sum())}")
