@@ -18,7 +18,7 @@ __host__ __device__ constexpr int fpa_waves_per_eu(int op, int K, int L) {
 #ifdef FPA_WAVES_PER_EU
   return (void)op, (void)K, (void)L, FPA_WAVES_PER_EU;
 #else
-  return op == 5 && K == 32 ? 4 : op == 4 && K == 16 && L == 8 ? 4 : 1;
+  return op == 5 && (K == 32 || (K == 16 && L == 8)) ? 4 : op == 4 && K == 16 && L == 8 ? 4 : 1;
 #endif
 }
 #define FPA_KERNEL_ATTRS(OP, K, L) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K, L))))

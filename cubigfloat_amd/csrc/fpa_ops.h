@@ -1754,7 +1754,7 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   double inv32 = 0.0;                      /* 2^32 / (T_{F-1} beta + T_{F-2}), from step 2 on */
   int32_t q0 = 0;
 
-  auto step = [&](int k, int idx, bool own, int kind) __attribute__((always_inline)) {
+  auto step = [&](int k, int idx, uint32_t ownm, int kind) __attribute__((always_inline)) {
     /* U <- U * beta */
     const int64_t so = C[KD - 1];
     const uint32_t inlo = g.from_prev_row0((uint32_t)so), inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
@@ -1776,17 +1776,18 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       }
     }
     q = (int32_t)g.bcast_top((uint32_t)q);
+    /* slot idx is still zero in the owner lane (ownm all ones; the lanes above it hold
+     * earlier digits there): the owner sets it to -q for the multiply-adds, so they subtract
+     * q^2 there too, then adds -q again for -2q (no branch on the owner lane) */
+    const uint32_t t = (uint32_t)(-q) & ownm;
+#pragma unroll
+    for (int i = 0; i < KD; i++)
+      if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
 #pragma unroll
     for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nT[i] + C[i];
-    if (own) {
 #pragma unroll
-      for (int i = 0; i < KD; i++) {
-        if (i == idx) {
-          C[i] -= (int64_t)q * q;
-          nT[i] = -2 * q;
-        }
-      }
-    }
+    for (int i = 0; i < KD; i++)
+      if (i == idx) nT[i] += (int32_t)t;
     if (kind == 0) q0 = q;
     if (kind == 1) {
       const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
@@ -1812,12 +1813,17 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   };
   wsync();   /* the recoding reads of lds are done before the digits overwrite them */
   /* trips of KD steps: step k's slot pos = F-1-k is index KD-1-u of lane L-1-trip */
+  /* ownm: all ones in the lane that owns the trip's slots; opaque, so it is formed once per
+   * trip rather than again at every step */
+  uint32_t ownm = top ? ~0u : 0u;
+  asm volatile("" : "+v"(ownm));
   repeat<KD>([&](int u) __attribute__((always_inline)) {
-    step(u, KD - 1 - u, top, u == 0 ? 0 : u == 1 ? 1 : 2);
+    step(u, KD - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2);
   });
   for (int tr = 1; tr < L; tr++) {
-    const bool own = g.gl == L - 1 - tr;
-    repeat<KD>([&](int u) __attribute__((always_inline)) { step(tr * KD + u, KD - 1 - u, own, 2); });
+    ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+    asm volatile("" : "+v"(ownm));
+    repeat<KD>([&](int u) __attribute__((always_inline)) { step(tr * KD + u, KD - 1 - u, ownm, 2); });
   }
 
   /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
