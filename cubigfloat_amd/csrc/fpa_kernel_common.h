@@ -35,8 +35,14 @@ struct ScratchWords {
                            : OP == 4 ? div_lds_words(K, L)
                            : sqrt_lazy_shape(K) && sqrt_lazy_lds_words<K, L>() > n + 1 ? sqrt_lazy_lds_words<K, L>()
                            : n + 1;
-  /* pad to a multiple of 4 words, plus 4 so successive values start on different banks */
-  static constexpr int value = ((raw + 3) / 4) * 4 + 4;
+  /* pad to a multiple of 4 words, plus 4 so successive values start on different banks.
+   * div and sqrt: an even stride of 2 mod 4 words instead, so the 64/L values of a wave
+   * start on 64/L different banks of the 32 (their digit steps and relayouts have every
+   * group access the same offset at once; a stride of 0 mod 32, which the rule above gives
+   * for some shapes, put all of them on one bank) */
+  static constexpr int quad = ((raw + 3) / 4) * 4 + 4;
+  static constexpr int even = ((raw + 1) / 2) * 2;
+  static constexpr int value = OP >= 4 ? (even % 4 == 2 ? even : even + 2) : quad;
 };
 
 /* Waves per block: 4, or 1 for mul with a short product, whose LDS then packs by the

@@ -1133,11 +1133,13 @@ __host__ __device__ constexpr int div_lds_words(int K, int L) {
                                                                    : div_steps(K * L) + 3;
 }
 
-/* LDS words per value of the lazy sqrt: recoding stage or the root digits (+ three zeros) */
+/* LDS words per value of the lazy sqrt: recoding stage, the radicand digits of the half-frame
+ * phase (with L zeros below and KH above), or the root digits (+ three zeros) */
 template <int K, int L>
 __host__ __device__ constexpr int sqrt_lazy_lds_words() {
-  return div_pad_words(K, L) + K * L + 2 > div_digits_per_lane(K) * L + 3 ? div_pad_words(K, L) + K * L + 2
-                                                                          : div_digits_per_lane(K) * L + 3;
+  constexpr int recode = div_pad_words(K, L) + K * L + 2, F = div_digits_per_lane(K) * L;
+  constexpr int half = L + F + (div_digits_per_lane(K) + 1) / 2, root = F + 3;
+  return recode > half ? (recode > root ? recode : root) : (half > root ? half : root);
 }
 
 /* x as held by group lane src, 64 bits */
@@ -1716,6 +1718,11 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  *      lower digits are below 2^-53 of it), V from the top two columns;
  *   3. U -= q * T (one v_mad_i64_i32 per column), the owner of slot pos subtracts q^2 and
  *      sets T's slot to 2q.
+ * In the first HF = ceil(KD/2)*L steps T fits the frame's top HF slots and the slots below
+ * only move radicand digits upwards: those steps run on a half frame of the top HF slots
+ * (ceil(KD/2) columns per lane), the radicand digits entering at its bottom one per step,
+ * and the frame is then laid out whole again through LDS (about 3/4 of the multiply-adds
+ * of F whole-frame steps).
  * The digits are redundant as in div and |q * 2q_j| < 2^57, so one carry level every 64
  * steps keeps the columns below 2^63.  At the end the estimate of R/T decides S unless it
  * lies within 2^-16 of 0 or 1; then R and T are normalised exactly and S moves by one
@@ -1724,6 +1731,77 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  * read from them.  The reference's root (fpa_warp_sqrt.h:195-254) runs 32-bit digits
  * with exact remainders instead.
  */
+/* Digit step k of the lazy sqrt on a frame of KC columns per lane (the whole frame, or the
+ * top half of it in the first phase).  U <- U * beta (kFill: 'fill', nonzero only in the
+ * group's lane 0, enters the bottom slot), digit q from the top two columns, U -= q * T,
+ * the owner lane's slot idx of T (zero so far) becomes 2q. */
+template <int KC, int L, bool kFill, bool kRenorm>
+__device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int k,
+                                               int idx, uint32_t ownm, int kind, bool top, uint32_t fill,
+                                               int32_t &q0, double &inv32) {
+  constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
+  constexpr double BETA_D = 134217728.0;   /* 2^27 */
+  /* U <- U * beta */
+  const int64_t so = C[KC - 1];
+  uint32_t inlo = g.from_prev_row0((uint32_t)so);
+  const uint32_t inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
+  if constexpr (kFill) inlo |= fill;
+#pragma unroll
+  for (int i = KC - 1; i > 0; i--) C[i] = C[i - 1];
+  C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
+  const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
+  C[KC - 1] = (int64_t)((uint64_t)C[KC - 1] + fold);
+  int32_t q;
+  if (kind == 0) {
+    q = (int32_t)fpa_isqrt64((uint64_t)C[KC - 1]);
+  } else {
+    const double Vs = __builtin_fma(i64_to_f64(C[KC - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KC - 2] >> 32));
+    if (kind == 1) {
+      const double a = (double)q0 * BETA_D, V = Vs * 0x1p32;
+      q = (int32_t)__builtin_floor(V / (a + __builtin_sqrt(__builtin_fma(a, a, V))));
+    } else {
+      q = (int32_t)__builtin_floor(Vs * inv32);
+    }
+  }
+  q = (int32_t)g.bcast_top((uint32_t)q);
+  /* slot idx is still zero in the owner lane (ownm all ones; the lanes above it hold
+   * earlier digits there): the owner sets it to -q for the multiply-adds, so they subtract
+   * q^2 there too, then adds -q again for -2q (no branch on the owner lane) */
+  const uint32_t t = (uint32_t)(-q) & ownm;
+#pragma unroll
+  for (int i = 0; i < KC; i++)
+    if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
+#pragma unroll
+  for (int i = 0; i < KC; i++) C[i] = (int64_t)q * nT[i] + C[i];
+#pragma unroll
+  for (int i = 0; i < KC; i++)
+    if (i == idx) nT[i] += (int32_t)t;
+  if (kind == 0) q0 = q;
+  if (kind == 1) {
+    const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
+    inv32 = 0x1p32 / tt;
+  }
+  /* keep the scheduler from running later steps' digit chains ahead of this step's
+   * multiply-adds (that holds several steps' columns live and costs occupancy) */
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (kRenorm) {
+    /* one carry level every 64 steps keeps the columns far below 2^63 */
+    if ((k & 63) == 63) {
+      int64_t car[KC];
+#pragma unroll
+      for (int i = 0; i < KC; i++) car[i] = C[i] >> DIV_RADIX;
+      const uint32_t clo = g.from_prev_row0((uint32_t)car[KC - 1]);
+      const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KC - 1] >> 32));
+      const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
+#pragma unroll
+      for (int i = KC - 1; i >= 0; i--) {
+        const int64_t keep = (top && i == KC - 1) ? C[i] : (C[i] & (int64_t)MASK);
+        C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
+      }
+    }
+  }
+}
+
 template <int K, int L>
 __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
                                                const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]) {
@@ -1746,84 +1824,103 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   const int64_t er = e / 2;
   uint32_t ad[KD];
   recode_digits_top<K, L, KD>(g, lds, A, ad, o);
-  int64_t C[KD];
-  int32_t nT[KD];
-#pragma unroll
-  for (int i = 0; i < KD; i++) { C[i] = (int64_t)ad[i]; nT[i] = 0; }
+  constexpr int KH = (KD + 1) / 2, HF = KH * L, P2 = F - HF;
+  constexpr bool kRen = F > 64;
+  static_assert(KH >= 2, "the first trip holds the two special steps");
   const bool top = g.gl == L - 1;
   double inv32 = 0.0;                      /* 2^32 / (T_{F-1} beta + T_{F-2}), from step 2 on */
   int32_t q0 = 0;
 
-  auto step = [&](int k, int idx, uint32_t ownm, int kind) __attribute__((always_inline)) {
-    /* U <- U * beta */
-    const int64_t so = C[KD - 1];
-    const uint32_t inlo = g.from_prev_row0((uint32_t)so), inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
-#pragma unroll
-    for (int i = KD - 1; i > 0; i--) C[i] = C[i - 1];
-    C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
-    const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
-    C[KD - 1] = (int64_t)((uint64_t)C[KD - 1] + fold);
-    int32_t q;
-    if (kind == 0) {
-      q = (int32_t)fpa_isqrt64((uint64_t)C[KD - 1]);
-    } else {
-      const double Vs = __builtin_fma(i64_to_f64(C[KD - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KD - 2] >> 32));
-      if (kind == 1) {
-        const double a = (double)q0 * BETA_D, V = Vs * 0x1p32;
-        q = (int32_t)__builtin_floor(V / (a + __builtin_sqrt(__builtin_fma(a, a, V))));
-      } else {
-        q = (int32_t)__builtin_floor(Vs * inv32);
-      }
-    }
-    q = (int32_t)g.bcast_top((uint32_t)q);
-    /* slot idx is still zero in the owner lane (ownm all ones; the lanes above it hold
-     * earlier digits there): the owner sets it to -q for the multiply-adds, so they subtract
-     * q^2 there too, then adds -q again for -2q (no branch on the owner lane) */
-    const uint32_t t = (uint32_t)(-q) & ownm;
-#pragma unroll
-    for (int i = 0; i < KD; i++)
-      if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
-#pragma unroll
-    for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nT[i] + C[i];
-#pragma unroll
-    for (int i = 0; i < KD; i++)
-      if (i == idx) nT[i] += (int32_t)t;
-    if (kind == 0) q0 = q;
-    if (kind == 1) {
-      const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
-      inv32 = 0x1p32 / tt;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if constexpr (F > 64) {
-      /* one carry level every 64 steps keeps the columns far below 2^63 */
-      if ((k & 63) == 63) {
-        int64_t car[KD];
-#pragma unroll
-        for (int i = 0; i < KD; i++) car[i] = C[i] >> DIV_RADIX;
-        const uint32_t clo = g.from_prev_row0((uint32_t)car[KD - 1]);
-        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KD - 1] >> 32));
-        const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
-#pragma unroll
-        for (int i = KD - 1; i >= 0; i--) {
-          const int64_t keep = (top && i == KD - 1) ? C[i] : (C[i] & (int64_t)MASK);
-          C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
-        }
-      }
-    }
-  };
+  /* Phase 1, steps 0 .. HF-1: T fits the frame's top HF slots, and U's slots below them only
+   * carry radicand digits upwards, one per step.  These steps run on a half frame of the top
+   * HF = KH*L slots, KH columns per lane, the digits below entering at its bottom lane.
+   * LDS: [0, L) zeros, [L, L+F) the radicand's digits, [L+F, L+F+KH) zeros. */
+  uint32_t *dig = lds + L;
   wsync();   /* the recoding reads of lds are done before the digits overwrite them */
-  /* trips of KD steps: step k's slot pos = F-1-k is index KD-1-u of lane L-1-trip */
-  /* ownm: all ones in the lane that owns the trip's slots; opaque, so it is formed once per
-   * trip rather than again at every step */
-  uint32_t ownm = top ? ~0u : 0u;
-  asm volatile("" : "+v"(ownm));
-  repeat<KD>([&](int u) __attribute__((always_inline)) {
-    step(u, KD - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2);
-  });
-  for (int tr = 1; tr < L; tr++) {
-    ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+  lds[g.gl] = 0;
+#pragma unroll
+  for (int i = 0; i < KD; i++) dig[g.gl * KD + i] = ad[i];
+#pragma unroll
+  for (int q = 0; q < (KH + L - 1) / L; q++)
+    if (q * L + g.gl < KH) dig[F + q * L + g.gl] = 0;
+  wsync();
+  int64_t Ch[KH];
+  int32_t nTh[KH];
+#pragma unroll
+  for (int i = 0; i < KH; i++) { Ch[i] = (int64_t)dig[P2 + g.gl * KH + i]; nTh[i] = 0; }
+  {
+    /* digit P2-1-k enters the bottom lane at step k (zeros once k >= P2; they come from
+     * [0, L)); the other lanes read zeros */
+    uint32_t fl[KH];
+    auto fills = [&](int tr) __attribute__((always_inline)) {
+      const uint32_t *b = g.gl == 0 ? dig + (P2 - KH - tr * KH) : dig + F;
+#pragma unroll
+      for (int u = 0; u < KH; u++) fl[u] = b[KH - 1 - u];
+    };
+    /* trips of KH steps: step k's half-frame slot HF-1-k is index KH-1-u of lane L-1-trip;
+     * ownm: all ones in the lane that owns the trip's slots, opaque so that it is formed
+     * once per trip rather than again at every step */
+    fills(0);
+    uint32_t ownm = top ? ~0u : 0u;
     asm volatile("" : "+v"(ownm));
-    repeat<KD>([&](int u) __attribute__((always_inline)) { step(tr * KD + u, KD - 1 - u, ownm, 2); });
+    repeat<KH>([&](int u) __attribute__((always_inline)) {
+      sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, fl[u],
+                                         q0, inv32);
+    });
+    for (int tr = 1; tr < L; tr++) {
+      fills(tr);
+      ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+      asm volatile("" : "+v"(ownm));
+      repeat<KH>([&](int u) __attribute__((always_inline)) {
+        sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, fl[u], q0, inv32);
+      });
+    }
+  }
+  /* the half frame's slots P2 .. F-1 into the whole frame (KD columns per lane, zeros below
+   * P2), one 32-bit word per pass through LDS */
+  int64_t C[KD];
+  int32_t nT[KD];
+  {
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)Ch[i];
+#pragma unroll
+    for (int q = 0; q < (P2 + L - 1) / L; q++)
+      if (q * L + g.gl < P2) lds[q * L + g.gl] = 0;
+    wsync();
+    uint32_t lo[KD];
+#pragma unroll
+    for (int i = 0; i < KD; i++) lo[i] = lds[g.gl * KD + i];
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)((uint64_t)Ch[i] >> 32);
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KD; i++) C[i] = (int64_t)(((uint64_t)lds[g.gl * KD + i] << 32) | lo[i]);
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)nTh[i];
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KD; i++) nT[i] = (int32_t)lds[g.gl * KD + i];
+  }
+  /* Phase 2, steps HF .. F-1 on the whole frame: slot F-1-k runs from P2-1 down to 0; the
+   * first trip covers the RT slots of lane LT, then whole trips of KD for lanes LT-1 .. 0 */
+  {
+    constexpr int LT = (P2 - 1) / KD, RT = P2 - LT * KD;
+    uint32_t ownm = g.gl == LT ? ~0u : 0u;
+    asm volatile("" : "+v"(ownm));
+    repeat<RT>([&](int u) __attribute__((always_inline)) {
+      sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, 0u, q0, inv32);
+    });
+    for (int tr = LT - 1; tr >= 0; tr--) {
+      ownm = g.gl == tr ? ~0u : 0u;
+      asm volatile("" : "+v"(ownm));
+      const int k0 = HF + RT + (LT - 1 - tr) * KD;
+      repeat<KD>([&](int u) __attribute__((always_inline)) {
+        sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, 0u, q0, inv32);
+      });
+    }
   }
 
   /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
