@@ -10,15 +10,17 @@
 #include "fpa_ops.h"
 #include "fpa_shapes.h"
 
-/* Occupancy target (waves per SIMD) of an element-wise kernel.  sqrt with 32 limbs per
- * lane needs 129-140 VGPRs left alone, which allows 3 waves per SIMD; held to 128 it
- * does not spill and runs 4.  Others take the compiler's own choice (1 = no target),
- * unless a build sets one for every kernel (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
+/* Occupancy target (waves per SIMD) of an element-wise kernel.  div and sqrt with 16 limbs
+ * on 8 lanes need 130-140 VGPRs left alone (3 waves per SIMD); held to 128 they spill a
+ * little outside the digit loop and run faster at 4.  Others take the compiler's own
+ * choice (1 = no target): with 32 limbs per lane div and sqrt need about 240 VGPRs and run
+ * best at 2 waves per SIMD without spilling (held to 3 they spilled and ran 4% slower).
+ * A build can set one target for every kernel (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
 __host__ __device__ constexpr int fpa_waves_per_eu(int op, int K, int L) {
 #ifdef FPA_WAVES_PER_EU
   return (void)op, (void)K, (void)L, FPA_WAVES_PER_EU;
 #else
-  return op == 5 && (K == 32 || (K == 16 && L == 8)) ? 4 : op == 4 && K == 16 && L == 8 ? 4 : 1;
+  return (op == 4 || op == 5) && K == 16 && L == 8 ? 4 : 1;
 #endif
 }
 #define FPA_KERNEL_ATTRS(OP, K, L) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K, L))))
@@ -33,8 +35,7 @@ struct ScratchWords {
                            : OP <= 2 ? n + 1
                            : OP == 3 ? mul_op_lds_words<K, L>()
                            : OP == 4 ? div_lds_words(K, L)
-                           : sqrt_lazy_shape(K) && sqrt_lazy_lds_words<K, L>() > n + 1 ? sqrt_lazy_lds_words<K, L>()
-                           : n + 1;
+                           : sqrt_lazy_lds_words<K, L>();
   /* pad to a multiple of 4 words, plus 4 so successive values start on different banks.
    * div and sqrt: an even stride of 2 mod 4 words instead, so the 64/L values of a wave
    * start on 64/L different banks of the 32 (their digit steps and relayouts have every
@@ -99,15 +100,15 @@ constexpr bool op_has_shape(int op, int K) { return op != 3 || K <= FPA_MUL_MAX_
 
 /* Compilation parts.  An op's shapes can be split over several files that build in
  * parallel; shapes are dealt to parts heaviest first, each to the least-loaded part, by a
- * rough compile-time weight (sqrt with 32 limbs per lane: about a minute per shape; with
- * 16: about 20 s; div with 32: about 12 s; most others 4-10 s). */
+ * rough compile-time weight (div and sqrt with 24-32 limbs per lane: 10-20 s per shape;
+ * with 16: about 8 s; most others 4-6 s). */
 constexpr int compile_weight(int op, int K) {
-  return op == 5 ? (K == 32 ? 12 : K == 16 ? 3 : 1) : op == 4 ? (K == 32 ? 2 : 1) : 1;
+  return op >= 4 ? (K >= 24 ? 3 : K >= 16 ? 2 : 1) : 1;
 }
 constexpr int compile_part(int op, int K, int L, int nparts) {
   int load[16] = {};
   int part = -1;
-  for (int w = 12; w >= 1; w--) {
+  for (int w = 3; w >= 1; w--) {
     for (int i = 0; i < kNumShapes; i++) {
       if (!op_has_shape(op, kShapes[i].K) || compile_weight(op, kShapes[i].K) != w) continue;
       int best = 0;

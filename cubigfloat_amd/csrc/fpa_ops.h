@@ -14,11 +14,9 @@
  *   div      fpa_warp_div.h:127-201 -> op_div: long division in radix 2^27 with the
  *            remainder kept as lazy signed 64-bit column sums (one multiply-add per column
  *            and digit), FP64 digit estimates, redundant digits, exact end correction.
- *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root; up to 24
- *            limbs per lane in radix 2^27 on div's lazy column frame (core_sqrt_lazy),
- *            with 32 limbs per lane with 32-bit digits in a fixed frame:
- *            R <- R*2^32 - d*(2S + d*2^(32 pos)), the digit estimated from (R/2)/S's top
- *            words, add-back while negative (core_sqrt).
+ *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root in radix
+ *            2^27 on div's lazy column frame (core_sqrt_lazy), the first half of the
+ *            digits on a half frame.
  * Rounding uses fpa_round's decision (fpa_scalar.h) on the exact (round word, sticky)
  * pair; a carry out of the rounding increment becomes 0x80..0 one binade up.
  */
@@ -1053,43 +1051,7 @@ __device__ __forceinline__ void op_mul(const fpa_arguments &a, const Slot &s, co
   store_result(a, s, g.gl, re, R);
 }
 
-/* Multiply-subtract step shared by div and sqrt:
- *   (rtop : R) -= q * (ttop : T), with R, T chunks of n words and rtop signed.
- * rtop is kept by the group's top lane only (other lanes hold don't-care values): the
- * word a lane's chunk pushes out, `out`, is the top lane's own in that lane. */
-template <int K, int L>
-__device__ __forceinline__ void msub_step(const Group<L> &g, uint32_t (&R)[K], int64_t &rtop, uint32_t q,
-                                          const uint32_t (&T)[K], uint32_t ttop) {
-  uint32_t c = 0;
-  unsigned int bw = 0;
-#pragma unroll
-  for (int i = 0; i < K; i++) {
-    uint64_t p = (uint64_t)q * T[i] + c;
-    c = (uint32_t)(p >> 32);
-    unsigned int o;
-    R[i] = __builtin_subc(R[i], (uint32_t)p, bw, &o);
-    bw = o;
-  }
-  uint32_t out = c + bw;                     /* <= 2^32-1: high product word is <= 2^32-2 */
-  uint32_t bout = g.sub_lane_words(R, g.from_prev_row0(out));
-  rtop = rtop - (int64_t)out - (int64_t)bout - (int64_t)((uint64_t)q * ttop);
-}
-
-/* true in every lane of the group when the top lane's rtop is negative */
-template <int L>
-__device__ __forceinline__ bool top_negative(const Group<L> &g, int64_t rtop) {
-  return g.any(g.gl == L - 1 && rtop < 0);
-}
-
 /* -------------------------------- div -------------------------------- */
-#ifndef FPA_DIV_UNROLL
-#define FPA_DIV_UNROLL 16
-#endif
-__host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
-/* digit steps per loop trip in div and sqrt (a divisor of K, so the trips cover n = K*L
- * steps) */
-__host__ __device__ constexpr int div_steps_per_trip(int K) { return gcd_int(K, FPA_DIV_UNROLL); }
-
 /* ------------------- div with lazy radix-2^27 column sums ------------------- */
 /*
  * Long division in radix beta = 2^27 with the remainder kept as signed 64-bit column sums
@@ -1392,300 +1354,6 @@ __device__ __forceinline__ void op_div(const fpa_arguments &a, const Slot &s, co
 }
 
 /* -------------------------------- sqrt -------------------------------- */
-/* shapes whose first n/2 - 1 digit steps run on the frame's top half (K/2 words per lane):
- * the steps per loop trip must be a multiple of K/2, and the top lane must still hold two
- * words */
-__host__ __device__ constexpr bool sqrt_windowed(int K) {
-  return K >= 4 && K % 2 == 0 && div_steps_per_trip(K) % (K / 2) == 0;
-}
-
-/* One root digit (position pos) on R, KC words per lane holding frame words
- * [n - KC*L, n): R <- R*2^32 (in0 enters at group lane 0's bottom, the top word goes to
- * rtop), digit estimate from (R/2)/S's top words in the top lane, broadcast,
- * R -= d*(T + d*2^(32 pos)) with T = 2S read from LDS (Tl, frame positions), add-back
- * while negative, then T gets the digit. */
-template <int KC, int K, int L>
-__device__ __forceinline__ void sqrt_step(const Group<L> &g, uint32_t *Tl, int pos, uint32_t (&R)[KC], uint32_t in0,
-                                          int64_t &rtop, uint32_t dtop, uint32_t &v) {
-  constexpr int n = K * L, B = n - KC * L;
-  const uint32_t rt = R[KC - 1];
-  const uint32_t in = g.from_prev_row(R[KC - 1], in0);
-#pragma unroll
-  for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
-  R[0] = in;
-  rtop = (rtop << 32) | rt;
-  const uint32_t r1 = R[KC - 1], r2 = R[KC - 2];
-  const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
-  const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
-  const uint32_t u0 = (r1 << 31) | (r2 >> 1);
-  const uint32_t t1 = Tl[n - 1], t2 = Tl[n - 2];
-  const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
-  if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
-  uint32_t d;
-  if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
-  else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
-  d = g.bcast_top(d);
-  if (g.gl == 0) Tl[pos] = d;
-  wsync();
-  uint32_t Tr[KC];
-  lds_load(Tr, Tl + B + g.gl * KC);
-  msub_step<KC, L>(g, R, rtop, d, Tr, dtop);
-  if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
-    /* add back (the estimate exceeded the digit): rare, off the common path */
-    for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
-      /* R(d-1) = R(d) + (T + d*2^(32pos)) + (d-1)*2^(32pos) */
-      uint32_t c = g.add(R, R, Tr);
-      rtop += (int64_t)c + dtop;
-      d -= 1u;
-      uint32_t Y[KC];
-#pragma unroll
-      for (int i = 0; i < KC; i++) {
-        bool here = B + g.gl * KC + i == pos;
-        Y[i] = here ? d : 0u;
-        Tr[i] = here ? d : Tr[i];
-      }
-      c = g.add(R, R, Y);
-      rtop += (int64_t)c;
-    }
-  }
-  wsync();
-  /* T <- 2S with the new digit: T[pos] = 2d mod 2^32, its carry sets bit 0 of T[pos+1] */
-  if (g.gl == 0) {
-    Tl[pos] = d << 1;
-    Tl[pos + 1] |= d >> 31;
-  }
-  wsync();
-}
-
-/* sqrt_step with T's words [B, n) in registers (TC: KC words per lane, laid out like R)
- * instead of LDS.  The digit's word pos sits at index idx (a constant of the unrolled step)
- * of group lane (pos - B) / KC; the step touches no LDS, so it needs no wave syncs. */
-template <int KC, int K, int L>
-__device__ __forceinline__ void sqrt_step_regs(const Group<L> &g, int pos, int idx, uint32_t (&R)[KC],
-                                               uint32_t (&TC)[KC], uint32_t in0, int64_t &rtop, uint32_t dtop,
-                                               uint32_t &v) {
-  constexpr int n = K * L, B = n - KC * L;
-  const int owner = (pos - B) / KC;
-  const bool mine = g.gl == owner;
-  const uint32_t rt = R[KC - 1];
-  const uint32_t in = g.from_prev_row(R[KC - 1], in0);
-#pragma unroll
-  for (int i = KC - 1; i > 0; i--) R[i] = R[i - 1];
-  R[0] = in;
-  rtop = (rtop << 32) | rt;
-  /* the top lane holds R's and T's two leading words */
-  const uint32_t r1 = R[KC - 1], r2 = R[KC - 2];
-  const uint32_t u2 = (uint32_t)((uint64_t)rtop >> 1);
-  const uint32_t u1 = ((uint32_t)rtop << 31) | (r1 >> 1);
-  const uint32_t u0 = (r1 << 31) | (r2 >> 1);
-  const uint32_t t1 = TC[KC - 1], t2 = TC[KC - 2];
-  const uint32_t s1 = (dtop << 31) | (t1 >> 1), s0 = (t1 << 31) | (t2 >> 1);
-  if (pos >= n - 3) v = fpa_recip_3by2(s1, s0);
-  uint32_t d;
-  if (u2 > s1 || (u2 == s1 && u1 >= s0)) d = 0xFFFFFFFFu;
-  else d = fpa_div3by2(u2, u1, u0, s1, s0, v);
-  d = g.bcast_top(d);
-  uint32_t Tr[KC];
-#pragma unroll
-  for (int i = 0; i < KC; i++) Tr[i] = (i == idx && mine) ? d : TC[i];
-  msub_step<KC, L>(g, R, rtop, d, Tr, dtop);
-  if (__builtin_expect(__ballot(g.gl == L - 1 && rtop < 0) != 0, 0)) {
-    for (int guard = 0; top_negative(g, rtop) && guard < 6; guard++) {
-      uint32_t c = g.add(R, R, Tr);
-      rtop += (int64_t)c + dtop;
-      d -= 1u;
-      uint32_t Y[KC];
-#pragma unroll
-      for (int i = 0; i < KC; i++) {
-        const bool here = i == idx && mine;
-        Y[i] = here ? d : 0u;
-        Tr[i] = here ? d : Tr[i];
-      }
-      c = g.add(R, R, Y);
-      rtop += (int64_t)c;
-    }
-  }
-  /* T <- 2S with the new digit: word pos = 2d mod 2^32, its carry sets bit 0 of pos+1 */
-  const bool next_mine = idx + 1 < KC ? mine : g.gl == owner + 1;
-#pragma unroll
-  for (int i = 0; i < KC; i++) {
-    if (i == idx) TC[i] = mine ? d << 1 : TC[i];
-    if (i == (idx + 1) % KC) TC[i] |= next_mine ? d >> 31 : 0u;
-  }
-}
-
-/* lds: n+1 words: T = 2S (+ current digit) at [0, n+1).  The root is T/2 at the end (the
- * bit above T's n words is dtop), so no separate root array is kept: a smaller LDS
- * footprint lets more waves share a CU. */
-template <int K, int L>
-__device__ __forceinline__ void core_sqrt(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
-                          uint32_t &re, uint32_t (&Sr)[K]) {
-  static_assert(K >= 2, "the top lane must hold the two leading words after a shift");
-  constexpr int n = K * L;
-  uint32_t act = k_sqrt_tab[fpa_classify(ea)];
-  if (act != FPA_A_SQRT) {
-    re = act;
-#pragma unroll
-    for (int i = 0; i < K; i++) Sr[i] = 0;
-    return;
-  }
-  uint32_t *Tl = lds;
-  uint32_t R[K];
-  int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
-  uint32_t shift_in = 0;
-  if (e & 1) {
-    /* odd exponent: radicand A/2, the dropped bit enters with the first shift */
-    shift_in = (g.bcast(A[0], 0) & 1u) << 31;
-    uint32_t nx = g.from_next(A[0]);
-#pragma unroll
-    for (int i = 0; i < K - 1; i++) R[i] = (A[i] >> 1) | (A[i + 1] << 31);
-    R[K - 1] = (A[K - 1] >> 1) | (nx << 31);
-    e += 1;
-  } else {
-#pragma unroll
-    for (int i = 0; i < K; i++) R[i] = A[i];
-  }
-  const int64_t er = e / 2;
-#pragma unroll
-  for (int i = 0; i < K; i++) Tl[g.gl * K + i] = 0;
-  if (g.gl == 0) Tl[n] = 0;
-
-  /* first digit: integer root of the top two radicand words */
-  int64_t rtop;
-  uint32_t dtop;
-  {
-    uint32_t rt = g.template word<K, n - 1>(R);
-    uint32_t in = g.from_prev_row(R[K - 1], shift_in);
-#pragma unroll
-    for (int i = K - 1; i > 0; i--) R[i] = R[i - 1];
-    R[0] = in;
-    uint64_t u = ((uint64_t)rt << 32) | g.template word<K, n - 1>(R);
-    uint32_t d1 = fpa_isqrt64(u);
-    uint64_t rem = u - (uint64_t)d1 * d1;
-    if (g.gl == L - 1) R[K - 1] = (uint32_t)rem;
-    rtop = (int64_t)(rem >> 32);
-    dtop = d1 >> 31;
-    wsync();
-    if (g.gl == 0) Tl[n - 1] = d1 << 1;
-    wsync();
-  }
-  uint32_t v = 0;
-  constexpr int U = div_steps_per_trip(K);
-  if constexpr (sqrt_windowed(K)) {
-    /* Digits n-2 .. n/2 change only words [n/2, n) of the frame: the words below are
-     * radicand words moving up (then zeros).  These digits run on that top half spread
-     * over all L lanes, KH = K/2 words each (W: frame words n/2 + gl*KH + i), and each
-     * step's shift takes in the radicand word below from X (frame words gl*KH + i as they
-     * stood after the first digit).  Then the frame is rebuilt and digits n/2-1 .. 0 run
-     * on all of it. */
-    constexpr int KH = K / 2, NH = n / 2;
-    uint32_t W[KH], X[KH];
-    {
-      /* frame word q (lane q/K, index q%K) -> half-layout lane q/KH, index q%KH */
-      const int src = (L == 1) ? 0 : ((L + g.gl) >> 1);
-      const bool upper = ((L + g.gl) & 1) != 0;
-      const int srcx = g.gl >> 1;
-      const bool upperx = (g.gl & 1) != 0;
-#pragma unroll
-      for (int i = 0; i < KH; i++) {
-        if constexpr (L == 1) {
-          W[i] = R[KH + i];
-          X[i] = R[i];
-        } else {
-          const uint32_t wl = g.bcast(R[i], src), wh = g.bcast(R[KH + i], src);
-          const uint32_t xl = g.bcast(R[i], srcx), xh = g.bcast(R[KH + i], srcx);
-          W[i] = upper ? wh : wl;
-          X[i] = upperx ? xh : xl;
-        }
-      }
-    }
-    /* phase-1 step s (digit n-2-s): the word entering the half at its bottom is X word
-     * NH-1-s; with U a multiple of KH its index in the lane is fixed by s mod U */
-    /* T = 2S over the half stays in registers (TW, laid out like W) while these digits
-     * run: digit n-2-s lands at index (NH-2-s) mod KH, fixed by s mod U like X's */
-    uint32_t TW[KH];
-    lds_load(TW, Tl + NH + g.gl * KH);
-    auto step1 = [&](int s0, int u) __attribute__((always_inline)) {
-      const int e = NH - 1 - s0 - u;
-      const uint32_t in0 = g.bcast(X[(KH - 1 - (u % KH)) % KH], e / KH);
-      const int idx = ((NH - 2 - u) % KH + KH) % KH;
-      sqrt_step_regs<KH, K, L>(g, n - 2 - s0 - u, idx, W, TW, in0, rtop, dtop, v);
-    };
-    static_assert(U % KH == 0, "phase-1 entering words need fixed lane indices");
-    int s0 = 0;
-    for (; s0 + U <= NH - 1; s0 += U) repeat<U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
-    repeat<(NH - 1) % U>([&](int u) __attribute__((always_inline)) { step1(s0, u); });
-    lds_store(Tl + NH + g.gl * KH, TW);
-    wsync();
-    /* frame word q >= NH is W word q - NH; word NH-1 is X word 0 (the last radicand word
-     * below the half); words below it are zero */
-    {
-#pragma unroll
-      for (int i = 0; i < K; i++) {
-        const int q = g.gl * K + i;
-        const int sl = (q - NH) / KH;
-        const uint32_t w = g.bcast(W[i % KH], sl < 0 ? 0 : sl);
-        const uint32_t x0 = g.bcast(X[0], 0);
-        R[i] = q >= NH ? w : q == NH - 1 ? x0 : 0u;
-      }
-    }
-    int p1 = NH - 1;
-    /* whole-frame digits take K steps per trip when K = 32 (16 otherwise): with K dividing
-     * the steps per trip, digit pos lands at index (NH-1-u) mod K, so T stays in registers
-     * here too */
-    constexpr int U2 = K == 32 ? 32 : U;
-    if constexpr (U2 % K == 0) {
-      uint32_t TC[K];
-      lds_load(TC, Tl + g.gl * K);
-      auto step2 = [&](int u) __attribute__((always_inline)) {
-        sqrt_step_regs<K, K, L>(g, p1 - u, ((NH - 1 - u) % K + K) % K, R, TC, 0u, rtop, dtop, v);
-      };
-      for (; p1 + 1 >= U2; p1 -= U2) repeat<U2>(step2);
-      repeat<NH % U2>(step2);
-      wsync();
-      lds_store(Tl + g.gl * K, TC);
-      wsync();
-    } else {
-      for (; p1 + 1 >= U; p1 -= U)
-        repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
-      repeat<NH % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p1 - u, R, 0u, rtop, dtop, v); });
-    }
-  } else {
-    /* digits n-2 down to 0, U per trip (see div) */
-    int p0 = n - 2;
-    for (; p0 + 1 >= U; p0 -= U)
-      repeat<U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p0 - u, R, 0u, rtop, dtop, v); });
-    repeat<(n - 1) % U>([&](int u) __attribute__((always_inline)) { sqrt_step<K, K, L>(g, lds, p0 - u, R, 0u, rtop, dtop, v); });
-  }
-  /* root S = (dtop : T) / 2 (n words), remainder (rtop : R) = X - S^2 */
-  uint32_t S[K], D[K];
-  {
-    uint32_t T[K];
-    lds_load(T, Tl + g.gl * K);
-    const uint32_t up = g.from_next(T[0], dtop);    /* the top lane takes dtop */
-#pragma unroll
-    for (int i = 0; i < K - 1; i++) S[i] = __builtin_amdgcn_alignbit(T[i + 1], T[i], 1u);
-    S[K - 1] = __builtin_amdgcn_alignbit(up, T[K - 1], 1u);
-  }
-  rtop = (int64_t)(((uint64_t)g.bcast((uint32_t)((uint64_t)rtop >> 32), L - 1) << 32) |
-                   g.bcast((uint32_t)rtop, L - 1));
-  const uint32_t rem_nz = (rtop != 0 || g.any(or_all(R) != 0)) ? 1u : 0u;
-  uint32_t gt;
-  if (rtop > 0) gt = 1;
-  else {
-    uint32_t bo = g.sub(D, R, S);
-    gt = (!bo && g.any(or_all(D) != 0)) ? 1u : 0u;
-  }
-  wsync();
-  int64_t E = er + FPA_BIAS_LL;
-  if (round_mantissa<K, L>(g, S, mode, 0u, gt ? 0x80000000u : 0u, rem_nz)) E += 1;
-  re = pack_exp(E, 0u, 0, 0x7FFFFFFFll);
-  chunk_select(Sr, true, S, S);
-}
-
-/* The lazy-column root (below) for shapes with up to 24 limbs per lane; with 32 its
- * 38 64-bit columns and their digits spill, and the 32-bit frame root above is kept. */
-__host__ __device__ constexpr bool sqrt_lazy_shape(int K) { return K <= 24; }
 template <int K, int L>
 __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
                                                const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]);
@@ -1694,10 +1362,7 @@ template <int K, int L>
 __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
-  if constexpr (sqrt_lazy_shape(K))
-    core_sqrt_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
-  else
-    core_sqrt<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  core_sqrt_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
   store_result(a, s, g.gl, re, R);
 }
 
