@@ -811,16 +811,19 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
  * products per column, each below 2^54), round from that sum when its bits just below the
  * round word are not all equal, and form the full product only when they are.
  *
- * With chunks of KD digits, chunk pairs (alpha, beta) with alpha + beta >= L-1 hold every
- * product of columns >= KD*(L-1); on the guard anti-diagonal alpha + beta = L-2 only the
- * corner x + y >= G of each pair's KD x KD digit block reaches column T = KD*(L-2) + G.
- * That is L(L+1)/2 pairs plus L-1 small corners instead of L^2 pairs.  The pairs are not
+ * With chunks of KD digits, chunk pairs (alpha, beta) with alpha + beta >= F0 hold every
+ * product of columns >= KD*F0; on the guard anti-diagonal alpha + beta = F0-1 only the
+ * corner x + y >= G of each pair's KD x KD digit block reaches column T = KD*(F0-1) + G
+ * (G >= KD, so the anti-diagonal below it holds no column >= T).  F0 = L-1 when T lies
+ * above chunk L-1's start: L(L+1)/2 pairs plus L-1 small corners instead of L^2 pairs;
+ * lower for 32-lane groups whose chunks are short against the 97 bits below the round
+ * word (e.g. 4 limbs on 32 lanes at 4096 bits: F0 = 29, 589 pairs).  The pairs are not
  * spread evenly over the rounds of the product-scanning form (round r needs only lanes
- * gl >= L-1-r), so every lane here runs its own list (fpa_mul_sched.h): two segments of a
+ * gl >= F0-r), so every lane here runs its own list (fpa_mul_sched.h): two segments of a
  * fixed b chunk and a rising a row, the window of 2*KD column sums sliding one chunk per
  * round by register renaming, each finished chunk added into a column array in LDS
  * (64-bit LDS atomic adds: several lanes finish the same chunk).  Only the columns >= T are
- * kept (chunk L-1 from column T on, and the chunks above it); what falls below is dropped.
+ * kept (chunk F0 from column T on, and the chunks above it); what falls below is dropped.
  */
 template <int K, int L>
 struct MulShort {
@@ -828,16 +831,23 @@ struct MulShort {
   static constexpr int KD = mul_digits_per_lane(K), KDP = mul_digits_padded(K), ND = KD * L;
   /* the check window [p-97, p-33) must lie above the error: 27T + 36 <= p - 97 */
   static constexpr int TMAX = (p - 133) / MUL_RADIX;
-  /* corner threshold; 2KD-1 (no corner) once T clears the guard pairs' top column */
-  static constexpr int G = TMAX - KD * (L - 2) < 2 * KD - 1 ? TMAX - KD * (L - 2) : 2 * KD - 1;
-  static constexpr int T = KD * (L - 2) + G;
+  /* the lowest anti-diagonal of full chunk pairs: L-1, or lower for 32-lane groups whose
+   * chunks are short against the 97 bits below the round word (with fewer lanes the full
+   * product scan's DPP window shift beat the extra rounds' LDS atomics: 4x16 at 2048 bits
+   * ran 2.2x slower on the short product, 8x16 at 4096 1.6x; 4x32 and 8x32, whose full scan
+   * shifts through ds_bpermute, ran 25-30% faster) */
+  static constexpr int F0 = L == 32 && TMAX / KD < L - 1 ? TMAX / KD : L - 1;
+  /* corner threshold on anti-diagonal F0-1; 2KD-1 (no corner) once T clears the guard
+   * pairs' top column */
+  static constexpr int G = TMAX - KD * (F0 - 1) < 2 * KD - 1 ? TMAX - KD * (F0 - 1) : 2 * KD - 1;
+  static constexpr int T = KD * (F0 - 1) + G;
   static constexpr bool kCorner = G <= 2 * KD - 2;
-  /* a's digit chunks, then the 64-bit columns of chunks L-1 .. 2L-1 */
+  /* a's digit chunks, then the 64-bit columns of chunks F0 .. 2L-1 */
   static constexpr int COL0 = L * KDP;
-  static constexpr int NCOL = (L + 1) * KD;
+  static constexpr int NCOL = (2 * L - F0) * KD;
   static constexpr int words = COL0 + 2 * NCOL;
-  /* T >= KD*(L-1): the chunks below L-1 that are dropped hold only columns below T */
-  static constexpr bool ok = MulSched<L>::ok && G >= KD && ND < 512 && 2 * L <= 64;
+  /* T >= KD*F0: the chunks below F0 that are dropped hold only columns below T */
+  static constexpr bool ok = F0 >= 1 && MulSched<L, F0>::ok && G >= KD && ND < 512 && 2 * L <= 64;
 };
 
 template <int K, int L>
@@ -846,7 +856,7 @@ __host__ __device__ constexpr bool mul_short_ok() { return MulShort<K, L>::ok; }
 /* product rounds per lane: the short product's schedule, or L for the full scan */
 template <int K, int L>
 __host__ __device__ constexpr int mul_rounds() {
-  if constexpr (mul_short_ok<K, L>()) return MulSched<L>::R1 + MulSched<L>::R2;
+  if constexpr (mul_short_ok<K, L>()) return MulSched<L, MulShort<K, L>::F0>::R1 + MulSched<L, MulShort<K, L>::F0>::R2;
   else return L;
 }
 
@@ -858,17 +868,17 @@ __host__ __device__ constexpr int mul_op_lds_words() {
 }
 
 /* one chunk (KD columns) of the window into the column array if the chunk is kept; of
- * chunk L-1 only columns C0.. (those at or above T) */
-template <int L, int KD, int C0>
+ * chunk F0 only columns C0.. (those at or above T) */
+template <int F0, int KD, int C0>
 __device__ __forceinline__ void mul_emit(uint64_t *cols, int c, const uint64_t *w) {
-  if (c >= L - 1) {
+  if (c >= F0) {
     /* an opaque offset keeps the base in one register and the columns as the
      * instructions' immediate offsets (folded, the chunk's negative start costs an add per
      * column) */
-    int off = (c - (L - 1)) * KD;
+    int off = (c - F0) * KD;
     asm volatile("" : "+v"(off));
     uint64_t *dst = cols + off;
-    if (c >= L) {
+    if (c > F0) {
 #pragma unroll
       for (int i = 0; i < C0; i++) __hip_atomic_fetch_add(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
@@ -897,9 +907,9 @@ __device__ __forceinline__ void mul_load_row(const uint32_t *adig, int alpha, ui
 /* The lane's two segments: R1 rounds of b chunk beta1 with a rows alpha1.., then R2 rounds
  * of beta2 with rows alpha2...  Both b chunks are fetched up front and every round's a row
  * is read before the previous round's atomics are issued (which the compiler may not move
- * loads above), so the LDS latency overlaps a round of multiply-adds; chunk L-1 keeps its
+ * loads above), so the LDS latency overlaps a round of multiply-adds; chunk F0 keeps its
  * columns C0.. */
-template <int R1, int R2, int L, int KD, int KDP, int C0>
+template <int R1, int R2, int L, int F0, int KD, int KDP, int C0>
 __device__ __forceinline__ void mul_short_rounds(const Group<L> &g, const uint32_t *adig, uint64_t *cols,
                                                  const uint32_t (&B)[KD], int alpha1, int beta1, int alpha2,
                                                  int beta2) {
@@ -926,9 +936,9 @@ __device__ __forceinline__ void mul_short_rounds(const Group<L> &g, const uint32
       for (int j = 0; j < KD; j++) W[i + j] += (uint64_t)ar[i] * (seg1 ? Bc1[j] : Bc2[j]);
     }
     const int c = seg1 ? alpha1 + t + beta1 : alpha2 + (t - R1) + beta2;
-    mul_emit<L, KD, C0>(cols, c, W);
+    mul_emit<F0, KD, C0>(cols, c, W);
     if (t == R1 - 1 || t == R - 1) {
-      mul_emit<L, KD, C0>(cols, c + 1, W + KD);
+      mul_emit<F0, KD, C0>(cols, c + 1, W + KD);
 #pragma unroll
       for (int i = 0; i < 2 * KD; i++) W[i] = 0;
     } else {
@@ -982,13 +992,14 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
   for (int k = 0; k < KD; k++) lds[g.gl * KDP + k] = A[k];
   wsync();
 
-  const uint32_t sc = MulSched<L>::lane(g.gl);
-  constexpr int C0 = S::T - KD * (L - 1);
-  mul_short_rounds<MulSched<L>::R1, MulSched<L>::R2, L, KD, KDP, C0>(
+  using Sched = MulSched<L, S::F0>;
+  const uint32_t sc = Sched::lane(g.gl);
+  constexpr int C0 = S::T - KD * S::F0;
+  mul_short_rounds<Sched::R1, Sched::R2, L, S::F0, KD, KDP, C0>(
       g, lds, cols, B, (int)(sc & 31u), (int)((sc >> 5) & 31u), (int)((sc >> 10) & 31u), (int)((sc >> 15) & 31u));
   if constexpr (S::kCorner) {
-    /* the corner x + y >= G of guard pair (ca, cb): columns KD*(L-2) + x + y, i.e. column
-     * array entries x + y - KD of chunk L-1 */
+    /* the corner x + y >= G of guard pair (ca, cb): columns KD*(F0-1) + x + y, i.e. column
+     * array entries x + y - KD of chunk F0 */
     constexpr int X0 = S::G - (KD - 1), NC = KD - X0;
     const int ca = (int)((sc >> 20) & 31u), cb = (int)((sc >> 25) & 31u);
     uint32_t bc[NC];
@@ -1014,14 +1025,16 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
   }
   wsync();
 
-  /* lane gl: low chunk gl (kept only for gl = L-1), high chunk gl = chunk L+gl */
+  /* lane gl: low chunk gl (kept only for gl >= F0), high chunk gl = chunk L+gl */
   uint64_t cl[KD], ch[KD];
   {
-    const uint64_t *hrow = cols + (g.gl + 1) * KD;
+    const uint64_t *hrow = cols + (L - S::F0 + g.gl) * KD;
+    const bool low = g.gl >= S::F0;
+    const uint64_t *lrow = cols + (low ? g.gl - S::F0 : 0) * KD;
 #pragma unroll
     for (int i = 0; i < KD; i++) {
       ch[i] = hrow[i];
-      cl[i] = g.gl == L - 1 ? cols[i] : 0ull;
+      cl[i] = low ? lrow[i] : 0ull;
     }
   }
   bool amb;

@@ -1,18 +1,19 @@
 """Writes fpa_mul_sched.h: the lane schedules of the short (upper-half) product in mul.
 
 A value's operands are L digit chunks each (chunk = one lane's KD radix-2^27 digits).
-The upper half of the product needs every chunk pair (alpha, beta) with
-alpha + beta >= L - 1 (full pairs) and, on anti-diagonal alpha + beta = L - 2 (guard
-pairs), the corner of digit products whose column reaches the rounding region.  Each lane
+The upper part of the product needs every chunk pair (alpha, beta) with
+alpha + beta >= F0 (full pairs; F0 = L - 1 for most shapes, lower where the rounding
+region reaches further down) and, on anti-diagonal alpha + beta = F0 - 1 (guard pairs),
+the corner of digit products whose column reaches the rounding region.  Each lane
 runs two segments of R1 and R2 rounds (the same for every lane, so all lanes flush their
 windows together); a segment keeps one b chunk (beta) and walks alpha upward, so its
 column window slides one chunk per round.  Row beta's full pairs are alpha in
-[L-1-beta, L-1]; a row may be extended downward, first by its guard pair (then computed
-whole, so it needs no corner), then by pairs of lower anti-diagonals, whose columns the
-kernel drops.  A guard pair no segment covers gets a corner on one lane.
+[max(0, F0-beta), L-1]; a row may be extended downward, first by its guard pair (then
+computed whole, so it needs no corner), then by pairs of lower anti-diagonals, whose
+columns the kernel drops.  A guard pair no segment covers gets a corner on one lane.
 
 The schedule packs every row into pieces of R1 and R2 rounds, exactly L of each, with as
-few rounds R1 + R2 as it can (at least ceil((L+1)/2)), for groups of up to 8 lanes the fewest
+few rounds R1 + R2 as it can (at least the full pairs over L), for groups of up to 8 lanes the fewest
 lanes finishing the same chunk in the same round (their LDS atomics on one word run one
 after another; a chunk of n pairs over R rounds forces ceil(n/R)), then as many guard
 pairs computed whole as it can.
@@ -22,12 +23,12 @@ import sys
 import time
 
 
-def row_splits(L, beta, R1, R2):
+def row_splits(L, F0, beta, R1, R2):
     """ways to cover row beta: (extension e, [(piece length, alpha_start), ...])"""
     out = []
-    for e in range(0, L - beta):
-        length = beta + 1 + e
-        a0 = L - 1 - beta - e
+    s0 = max(0, F0 - beta)
+    for e in range(0, s0 + 1):
+        a0 = s0 - e
 
         def rec(pos, acc):
             if pos == L:
@@ -42,7 +43,7 @@ def row_splits(L, beta, R1, R2):
     return out
 
 
-def collisions(L, R1, R2, pieces):
+def collisions(L, F0, R1, R2, pieces):
     """sum over the emission instructions (each round's finished chunk, and the flush's
     second chunk) of the most lanes of a group adding into one kept chunk; the LDS runs
     same-word atomics of one instruction one after another"""
@@ -52,22 +53,22 @@ def collisions(L, R1, R2, pieces):
         for u in range(rk + 1):
             cnt = {}
             for d in starts:
-                if d + u >= L - 1:
+                if d + u >= F0:
                     cnt[d + u] = cnt.get(d + u, 0) + 1
             cost += max(cnt.values()) if cnt else 0
     return cost
 
 
-def guards_covered(L, pieces):
-    return {b for (b, a, m) in pieces if a + b <= L - 2 < a + b + m and b <= L - 2}
+def guards_covered(L, F0, pieces):
+    return {b for (b, a, m) in pieces if a + b <= F0 - 1 < a + b + m and b <= F0 - 1}
 
 
-def solve(L, R1, R2, budget_s=20.0):
+def solve(L, F0, R1, R2, budget_s=20.0):
     """Row pieces, exactly L of length R1 and L of length R2.  Small groups: exhaustive
     search for the fewest atomic collisions, then the most guard pairs whole.  Large groups:
     the most guard pairs whole (dynamic programme over the rows), collisions not ranked.
     Returns [(beta, alpha_start, length), ...] or None."""
-    splits = [row_splits(L, b, R1, R2) for b in range(L)]
+    splits = [row_splits(L, F0, b, R1, R2) for b in range(L)]
     if L <= 8:
         best = [None, None]
         t0 = time.time()
@@ -77,7 +78,7 @@ def solve(L, R1, R2, budget_s=20.0):
                 return
             if beta < 0:
                 if n1 == L and n2 == L:
-                    key = (collisions(L, R1, R2, pieces), -len(guards_covered(L, pieces)))
+                    key = (collisions(L, F0, R1, R2, pieces), -len(guards_covered(L, F0, pieces)))
                     if best[1] is None or key < best[1]:
                         best[0], best[1] = list(pieces), key
                 return
@@ -107,7 +108,7 @@ def solve(L, R1, R2, budget_s=20.0):
             r = best_from(beta + 1, u1 + k1, u2 + len(ps) - k1)
             if r is None:
                 continue
-            gd = 1 if (e >= 1 and beta <= L - 2) else 0
+            gd = 1 if (e >= 1 and beta <= F0 - 1) else 0
             if top is None or r[0] + gd > top[0]:
                 top = (r[0] + gd, [(beta, a, m) for m, a in ps] + r[1])
         memo[key] = top
@@ -117,18 +118,22 @@ def solve(L, R1, R2, budget_s=20.0):
     return None if r is None else r[1]
 
 
-def schedule(L):
-    R = (L + 2) // 2
+def full_pairs(L, F0):
+    return sum(1 for a in range(L) for b in range(L) if a + b >= F0)
+
+
+def schedule(L, F0):
+    R = -(-full_pairs(L, F0) // L)
     while True:
         found = None
         for R2 in range(1, R // 2 + 1):
             R1 = R - R2
             if R1 == R2:
                 continue
-            pieces = solve(L, R1, R2)
+            pieces = solve(L, F0, R1, R2)
             if pieces is None:
                 continue
-            key = (collisions(L, R1, R2, pieces), -len(guards_covered(L, pieces)))
+            key = (collisions(L, F0, R1, R2, pieces), -len(guards_covered(L, F0, pieces)))
             if found is None or key < found[0]:
                 found = (key, R1, R2, pieces)
         if found:
@@ -136,12 +141,12 @@ def schedule(L):
         R += 1
 
 
-def lanes(L, R1, R2, pieces):
+def lanes(L, F0, R1, R2, pieces):
     p1 = sorted((a, b) for (b, a, m) in pieces if m == R1)
     p2 = sorted((a, b) for (b, a, m) in pieces if m == R2)
     assert len(p1) == L and len(p2) == L
-    covered = guards_covered(L, pieces)
-    corners = [(L - 2 - beta, beta) for beta in range(L - 1) if beta not in covered]
+    covered = guards_covered(L, F0, pieces)
+    corners = [(F0 - 1 - beta, beta) for beta in range(max(0, F0 - L), F0) if beta not in covered]
     assert len(corners) <= L
     out = []
     for j in range(L):
@@ -151,24 +156,29 @@ def lanes(L, R1, R2, pieces):
     return out, len(corners)
 
 
-def check(L, R1, R2, table):
+def check(L, F0, R1, R2, table):
     """every full pair exactly once, no guard pair twice, rows inside [0, L)"""
     seen = {}
     for (a1, b1, a2, b2, ca, cb, has) in table:
         for (a, b, m) in ((a1, b1, R1), (a2, b2, R2)):
             for u in range(m):
                 assert 0 <= a + u < L and 0 <= b < L
-                if a + u + b >= L - 2:
+                if a + u + b >= F0 - 1:
                     key = (a + u, b)
                     assert key not in seen, key
                     seen[key] = 1
         if has:
-            assert ca + cb == L - 2 and (ca, cb) not in seen
+            assert ca + cb == F0 - 1 and (ca, cb) not in seen
             seen[(ca, cb)] = 1
     for a in range(L):
         for b in range(L):
-            if a + b >= L - 2:
+            if a + b >= F0 - 1:
                 assert (a, b) in seen, (L, a, b)
+
+
+# (L, F0) pairs the compiled shapes use: F0 = L - 1, and for L = 32 min(31, TMAX // KD)
+# with TMAX = (p - 133) // 27 (MulShort in fpa_ops.h)
+PAIRS = ((4, 3), (8, 7), (16, 15), (32, 31), (32, 30), (32, 29), (32, 28), (32, 27))
 
 
 def main():
@@ -176,29 +186,33 @@ def main():
     out.write("/* Generated by gen_mul_sched.py: lane schedules of the short product in mul.\n"
               " * Per lane one 32-bit word: bits 0-4 alpha of segment 1's first round, 5-9 its beta,\n"
               " * 10-14 / 15-19 the same for segment 2, 20-24 / 25-29 the corner's guard pair\n"
-              " * (alpha, beta), bit 30 set when the lane has a corner. */\n")
+              " * (alpha, beta), bit 30 set when the lane has a corner.  MulSched<L, F0>: groups of L\n"
+              " * lanes, full chunk pairs on anti-diagonals >= F0, guard pairs on F0 - 1. */\n")
     out.write("#ifndef FPA_MUL_SCHED_H\n#define FPA_MUL_SCHED_H\n\n#include <stdint.h>\n\nnamespace fpa {\n\n")
-    out.write("template <int L> struct MulSched { static constexpr bool ok = false; };\n\n")
-    for L in (4, 8, 16, 32):
-        R1, R2, pieces = schedule(L)
-        table, ncorner = lanes(L, R1, R2, pieces)
-        check(L, R1, R2, table)
+    out.write("template <int L, int F0> struct MulSched { static constexpr bool ok = false; };\n\n")
+    for L, F0 in PAIRS:
+        R1, R2, pieces = schedule(L, F0)
+        table, ncorner = lanes(L, F0, R1, R2, pieces)
+        check(L, F0, R1, R2, table)
         words = []
         for (a1, b1, a2, b2, ca, cb, has) in table:
             words.append(a1 | b1 << 5 | a2 << 10 | b2 << 15 | ca << 20 | cb << 25 | has << 30)
-        out.write(f"/* L = {L}: {R1} + {R2} rounds per lane ({L * (R1 + R2)} slots for "
-                  f"{L * (L + 1) // 2} full pairs), {L - 1 - ncorner} guard pairs whole, "
-                  f"{ncorner} corners; {collisions(L, R1, R2, pieces)} lane-chunk collisions over "
+        nguard = sum(1 for b in range(max(0, F0 - L), F0))
+        out.write(f"/* L = {L}, F0 = {F0}: {R1} + {R2} rounds per lane ({L * (R1 + R2)} slots for "
+                  f"{full_pairs(L, F0)} full pairs), {nguard - ncorner} guard pairs whole, "
+                  f"{ncorner} corners; {collisions(L, F0, R1, R2, pieces)} lane-chunk collisions over "
                   f"{R1 + R2 + 2} emissions */\n")
-        out.write(f"__constant__ uint32_t k_mul_sched_{L}[{L}] = {{\n")
+        out.write(f"__constant__ uint32_t k_mul_sched_{L}_{F0}[{L}] = {{\n")
         for i in range(0, L, 8):
             out.write("  " + ", ".join(f"0x{w:08x}u" for w in words[i:i + 8]) + ",\n")
         out.write("};\n")
-        out.write(f"template <> struct MulSched<{L}> {{\n"
-                  f"  static constexpr bool ok = true;\n"
+        ok = "true" if R1 + R2 < L else "false"
+        out.write(f"template <> struct MulSched<{L}, {F0}> {{\n"
+                  f"  static constexpr bool ok = {ok};\n"
                   f"  static constexpr int R1 = {R1}, R2 = {R2}, NCORNER = {ncorner};\n"
-                  f"  __device__ static uint32_t lane(int gl) {{ return k_mul_sched_{L}[gl]; }}\n"
+                  f"  __device__ static uint32_t lane(int gl) {{ return k_mul_sched_{L}_{F0}[gl]; }}\n"
                   f"}};\n\n")
+        print(f"L={L} F0={F0}: R1={R1} R2={R2} corners={ncorner}", file=sys.stderr, flush=True)
     out.write("}  // namespace fpa\n\n#endif\n")
 
 

@@ -3,9 +3,9 @@
 For every compiled mul shape with a short product, the lane schedules of
 cubigfloat_amd/csrc/fpa_mul_sched.h are replayed on Python integers exactly as the kernel
 sums them: each lane's two segments of chunk pairs (window of 2*KD columns, chunks below
-L-1 dropped) plus the guard corners.  Columns below T are dropped, as in the kernel.  Pinned here:
-  * the schedule covers every chunk pair with alpha + beta >= L-1 exactly once, and no pair
-    of the guard anti-diagonal twice;
+F0 dropped, chunk F0 from column T on) plus the guard corners.  Pinned here:
+  * the schedule covers every chunk pair with alpha + beta >= F0 exactly once, and no pair
+    of the guard anti-diagonal F0-1 twice;
   * Q <= A*B < Q + 2^(27T+36) for random, all-ones and alphabet operands (the error bound
     the kernel's decision rests on);
   * whenever the 64 bits of Q below the round word are neither all zeros nor all ones,
@@ -34,13 +34,14 @@ def shapes():
 
 
 def schedules():
+    """{(L, F0): (ok, R1, R2, lane words)}"""
     txt = open(os.path.join(CSRC, "fpa_mul_sched.h")).read()
     out = {}
-    for m in re.finditer(r"k_mul_sched_(\d+)\[\d+\] = \{([^}]*)\}", txt):
-        L = int(m.group(1))
-        words = [int(w, 16) for w in re.findall(r"0x([0-9a-f]+)u", m.group(2))]
-        r = re.search(r"struct MulSched<%d> \{[^}]*R1 = (\d+), R2 = (\d+)" % L, txt)
-        out[L] = (int(r.group(1)), int(r.group(2)), words)
+    for m in re.finditer(r"k_mul_sched_(\d+)_(\d+)\[\d+\] = \{([^}]*)\}", txt):
+        L, F0 = int(m.group(1)), int(m.group(2))
+        words = [int(w, 16) for w in re.findall(r"0x([0-9a-f]+)u", m.group(3))]
+        r = re.search(r"struct MulSched<%d, %d> \{[^}]*ok = (true|false);[^}]*R1 = (\d+), R2 = (\d+)" % (L, F0), txt)
+        out[(L, F0)] = (r.group(1) == "true", int(r.group(2)), int(r.group(3)), words)
     return out
 
 
@@ -52,10 +53,11 @@ def params(K, L):
     ND = KD * L
     p = 32 * K * L
     tmax = (p - 133) // RADIX
-    G = min(tmax - KD * (L - 2), 2 * KD - 1)
-    T = KD * (L - 2) + G
-    ok = L in SCHED and G >= KD and ND < 512 and 2 * L <= 64
-    return dict(KD=KD, ND=ND, p=p, G=G, T=T, ok=ok)
+    F0 = min(L - 1, tmax // KD) if L == 32 else L - 1
+    G = min(tmax - KD * (F0 - 1), 2 * KD - 1)
+    T = KD * (F0 - 1) + G
+    ok = (L, F0) in SCHED and SCHED[(L, F0)][0] and F0 >= 1 and G >= KD and ND < 512 and 2 * L <= 64
+    return dict(KD=KD, ND=ND, p=p, F0=F0, G=G, T=T, ok=ok)
 
 
 SHORT = [(n, K, L) for (n, K, L) in shapes() if params(K, L)["ok"]]
@@ -68,17 +70,17 @@ def digits(x, ND):
 def short_q(A, B, K, L):
     """the kernel's sum: columns as integers, then Q = sum col * 2^(27 s)"""
     P = params(K, L)
-    KD, ND, G = P["KD"], P["ND"], P["G"]
-    R1, R2, words = SCHED[L]
+    KD, ND, G, F0 = P["KD"], P["ND"], P["G"], P["F0"]
+    _, R1, R2, words = SCHED[(L, F0)]
     a, b = digits(A, ND), digits(B, ND)
     cols = {}
 
-    T = P["T"]
+    C0 = P["T"] - KD * F0
 
     def emit(c, w):
-        if c >= L - 1:
+        if c >= F0:
             for i in range(KD):
-                if c * KD + i >= T:       # chunk L-1 keeps its columns from T on
+                if c > F0 or i >= C0:     # chunk F0 keeps its columns from T on
                     cols[c * KD + i] = cols.get(c * KD + i, 0) + w[i]
 
     for sc in words:
@@ -97,11 +99,11 @@ def short_q(A, B, K, L):
                     W = W[KD:] + [0] * KD
         if (sc >> 30) & 1:
             ca, cb = (sc >> 20) & 31, (sc >> 25) & 31
-            assert ca + cb == L - 2
+            assert ca + cb == F0 - 1
             for x in range(KD):
                 for y in range(KD):
                     if x + y >= G:
-                        s = KD * (L - 2) + x + y
+                        s = KD * (F0 - 1) + x + y
                         cols[s] = cols.get(s, 0) + a[ca * KD + x] * b[cb * KD + y]
     for s, v in cols.items():
         assert v < 1 << 64, "column sum overflows 64 bits"
@@ -168,7 +170,7 @@ def operands(p, rng):
 
 
 def test_schedule_covers_each_pair_once():
-    for L, (R1, R2, words) in SCHED.items():
+    for (L, F0), (ok, R1, R2, words) in SCHED.items():
         assert len(words) == L
         seen = {}
         for sc in words:
@@ -176,19 +178,19 @@ def test_schedule_covers_each_pair_once():
                 for u in range(rs):
                     al = a0 + u
                     assert 0 <= al < L and 0 <= b0 < L
-                    if al + b0 >= L - 2:
+                    if al + b0 >= F0 - 1:
                         assert (al, b0) not in seen
                         seen[(al, b0)] = "whole"
             if (sc >> 30) & 1:
                 key = ((sc >> 20) & 31, (sc >> 25) & 31)
-                assert sum(key) == L - 2 and key not in seen
+                assert sum(key) == F0 - 1 and key not in seen
                 seen[key] = "corner"
         for al in range(L):
             for be in range(L):
-                if al + be >= L - 2:
-                    assert (al, be) in seen, (L, al, be)
-        # fewer rounds than the L of the full product scan
-        assert R1 + R2 < L
+                if al + be >= F0 - 1:
+                    assert (al, be) in seen, (L, F0, al, be)
+        # a usable schedule has fewer rounds than the L of the full product scan
+        assert ok == (R1 + R2 < L)
 
 
 def test_short_shapes_exist():
@@ -200,7 +202,7 @@ def test_short_shapes_exist():
 def test_short_product_bound_and_rounding(n, K, L):
     P = params(K, L)
     p, T = P["p"], P["T"]
-    assert T >= P["KD"] * (L - 1) and RADIX * T + 36 <= p - 97
+    assert T >= P["KD"] * P["F0"] and RADIX * T + 36 <= p - 97
     rng = random.Random(1000 * K + L)
     for A, B in operands(p, rng):
         Q = short_q(A, B, K, L)
