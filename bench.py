@@ -425,7 +425,7 @@ def main():
     wanted_cfg = set(c for c in args.configs.split(",") if c)
     if "3" in wanted_cfg:
         print("[bench] config 3: mixed batch, one graph launch", file=sys.stderr, flush=True)
-        configs["config3"] = bench_config3(F, hip, 1 << 18, 5, not args.no_cpu, threads)
+        configs["config3"] = bench_config3(F, hip, 1 << 18, 20, not args.no_cpu, threads)
     if "4" in wanted_cfg:
         print("[bench] config 4: dot chains", file=sys.stderr, flush=True)
         configs["config4"] = bench_config4(F, hip, 16384, 3, not args.no_cpu, threads)

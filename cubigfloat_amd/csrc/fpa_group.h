@@ -339,6 +339,16 @@ struct Group {
     else return from_prev_row(x, 0u);
   }
 
+  /* x of group lane gl+1 and zero in the group's top lane, through DPP row_shl:1 for groups
+   * inside one 16-lane row (the row's lane 15 reads zero through bound_ctrl; a smaller
+   * group's top lane, which reads the next group's lane 0, is selected to zero) */
+  __device__ __forceinline__ uint32_t from_next_row0(uint32_t x) const {
+    static_assert(L >= 2 && L <= 16, "groups inside one DPP row");
+    const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
+    if constexpr (L == 16) return y;
+    else return gl == L - 1 ? 0u : y;
+  }
+
   /* x of group lane gl-1; 'fill' in group lane 0 */
   __device__ __forceinline__ uint32_t from_prev(uint32_t x, uint32_t fill = 0) const {
     if constexpr (L == 1) return fill;

@@ -432,6 +432,40 @@ __device__ __forceinline__ void recode_digits(const Group<L> &g, uint32_t *lds, 
   wsync();
 }
 
+/* recode_digits without LDS, for shapes whose lanes' digit windows drift by less than a
+ * word over the group: lane gl's digits are bits [27*KD*gl, +27*KD) = the lane's own words
+ * from bit E*gl on (E = 27*KD - 32K) plus the next lane's first words, which come through
+ * a DPP row shift (zero above the value).  16 limbs per lane: E = 1, two words of the next
+ * lane. */
+template <int K, int L, int KD>
+__host__ __device__ constexpr bool recode_dpp_ok() {
+  return L >= 2 && L <= 16 && (MUL_RADIX * KD - 32 * K) * (L - 1) < 32 &&
+         (MUL_RADIX * KD + 31) / 32 + 1 - K <= K;
+}
+template <int K, int L, int KD>
+__device__ __forceinline__ void recode_digits_dpp(const Group<L> &g, const uint32_t (&X)[K], uint32_t (&D)[KD]) {
+  constexpr int E = MUL_RADIX * KD - 32 * K;
+  constexpr int NW = (MUL_RADIX * KD + 31) / 32;   /* window words */
+  constexpr int NX = NW + 1 - K;                   /* words taken from the next lane */
+  constexpr uint32_t M = (1u << MUL_RADIX) - 1;
+  static_assert(recode_dpp_ok<K, L, KD>(), "digit window within the lane and the next one");
+  uint32_t w[K + NX];
+#pragma unroll
+  for (int i = 0; i < K; i++) w[i] = X[i];
+#pragma unroll
+  for (int j = 0; j < NX; j++) w[K + j] = g.from_next_row0(X[j]);
+  const uint32_t s = (uint32_t)(E * g.gl);
+  uint32_t win[NW + 1];
+#pragma unroll
+  for (int i = 0; i < NW; i++) win[i] = __builtin_amdgcn_alignbit(w[i + 1], w[i], s);
+  win[NW] = 0;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    const int j = (MUL_RADIX * k) >> 5, sh = (MUL_RADIX * k) & 31;
+    D[k] = (sh + MUL_RADIX <= 32 ? win[j] >> sh : __builtin_amdgcn_alignbit(win[j + 1], win[j], (uint32_t)sh)) & M;
+  }
+}
+
 /* local radix-2^27 normalisation of KD column sums with carry-in c; returns the carry out */
 template <int KD>
 __device__ __forceinline__ uint64_t normalize_columns(const uint64_t (&col)[KD], uint32_t (&dg)[KD], uint64_t c) {
@@ -986,8 +1020,13 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
     }
   }
   uint32_t A[KD], B[KD];
-  recode_digits<K, L, KD>(g, lds, Aw, A);
-  recode_digits<K, L, KD>(g, lds, Bw, B);
+  if constexpr (recode_dpp_ok<K, L, KD>()) {
+    recode_digits_dpp<K, L, KD>(g, Aw, A);
+    recode_digits_dpp<K, L, KD>(g, Bw, B);
+  } else {
+    recode_digits<K, L, KD>(g, lds, Aw, A);
+    recode_digits<K, L, KD>(g, lds, Bw, B);
+  }
 #pragma unroll
   for (int k = 0; k < KD; k++) lds[g.gl * KDP + k] = A[k];
   wsync();
