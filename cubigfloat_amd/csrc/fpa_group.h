@@ -315,6 +315,24 @@ struct Group {
     } else return bcast(x, L - 1);
   }
 
+  /* x as held by group lane J (a constant) through DPP moves for groups inside one row:
+   * quad_perm for 2 and 4 lanes, row_newbcast:J for 16, and for 8 row_newbcast:8+J to the
+   * whole row then row_newbcast:J written to banks 0 and 1 (row lanes 0-7); ds_bpermute for
+   * 32 and 64 */
+  template <int J>
+  __device__ __forceinline__ uint32_t bcast_lane(uint32_t x) const {
+    static_assert(J >= 0 && J < L, "a lane of the group");
+    if constexpr (L == 1) return x;
+    else if constexpr (L == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, J | J << 2 | (2 + J) << 4 | (2 + J) << 6, 0xF, 0xF, false);
+    else if constexpr (L == 4) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, J * 0x55, 0xF, 0xF, false);
+    else if constexpr (L == 8) {
+      const int t = __builtin_amdgcn_mov_dpp((int)x, 0x158 + J, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_update_dpp(t, (int)x, 0x150 + J, 0xF, 0x3, false);
+    } else if constexpr (L == 16) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150 + J, 0xF, 0xF, false);
+    } else return bcast(x, J);
+  }
+
   /* from_prev through a DPP move for the digit steps' serial path, instead of a ds_bpermute
    * round trip: a row shift (row_shr:1) for groups inside one 16-lane row, a wave shift
    * (wave_shr:1) for groups of 32.  Lane 0 of a group takes 'fill' (the shift gave it the
@@ -339,14 +357,23 @@ struct Group {
     else return from_prev_row(x, 0u);
   }
 
-  /* x of group lane gl+1 and zero in the group's top lane, through DPP row_shl:1 for groups
-   * inside one 16-lane row (the row's lane 15 reads zero through bound_ctrl; a smaller
-   * group's top lane, which reads the next group's lane 0, is selected to zero) */
+  /* x of group lane gl+1, 'fill' in the group's top lane: DPP row_shl:1 for groups inside
+   * one 16-lane row (a group's top lane reads the next group's lane 0, or zero at the row's
+   * end, and takes 'fill' instead), ds_bpermute for larger groups */
+  __device__ __forceinline__ uint32_t from_next_row(uint32_t x, uint32_t fill = 0) const {
+    if constexpr (L >= 2 && L <= 16) {
+      const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
+      return gl == L - 1 ? fill : y;
+    } else {
+      return from_next(x, fill);
+    }
+  }
+
+  /* from_next_row with a zero fill: a 16-lane group's top lane is the row's lane 15, which
+   * row_shl:1 already gives a zero (bound_ctrl), so no select is needed there */
   __device__ __forceinline__ uint32_t from_next_row0(uint32_t x) const {
-    static_assert(L >= 2 && L <= 16, "groups inside one DPP row");
-    const uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
-    if constexpr (L == 16) return y;
-    else return gl == L - 1 ? 0u : y;
+    if constexpr (L == 16) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
+    else return from_next_row(x, 0u);
   }
 
   /* x of group lane gl-1; 'fill' in group lane 0 */
@@ -452,7 +479,7 @@ struct Group {
   /* limb 'pos' (compile-time) of the group's value */
   template <int K, int POS>
   __device__ __forceinline__ uint32_t word(const uint32_t (&x)[K]) const {
-    return bcast(x[POS % K], POS / K);
+    return bcast_lane<POS / K>(x[POS % K]);
   }
 
   /* number of leading zero bits of the n = K*L word value (32n if zero) */

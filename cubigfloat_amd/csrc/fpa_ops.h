@@ -100,7 +100,7 @@ struct Slot {
 template <int K, int L>
 __device__ __forceinline__ uint32_t round_mantissa(const Group<L> &g, uint32_t (&x)[K], int mode, uint32_t sign,
                                                    uint32_t rw, uint32_t sticky) {
-  uint32_t lsb = g.bcast(x[0], 0) & 1u;
+  uint32_t lsb = g.template bcast_lane<0>(x[0]) & 1u;
   uint32_t inc = fpa_round_inc(mode, sign, lsb, rw, sticky);
   uint32_t cout = g.add_lane_words(x, g.gl == 0 ? inc : 0u);
   if (cout) {
@@ -241,8 +241,8 @@ __device__ __forceinline__ void core_addsub(const Group<L> &g, uint32_t *lds, in
     E = Ex;
     if (c) {
       /* shift right one bit, the carry becomes the top bit */
-      uint32_t low = g.bcast(T[0], 0) & 1u;
-      uint32_t nxt = g.from_next(T[0], c);
+      uint32_t low = g.template bcast_lane<0>(T[0]) & 1u;
+      uint32_t nxt = g.from_next_row(T[0], c);
 #pragma unroll
       for (int i = 0; i < K - 1; i++) T[i] = (T[i] >> 1) | (T[i + 1] << 31);
       T[K - 1] = (T[K - 1] >> 1) | (nxt << 31);
@@ -255,7 +255,7 @@ __device__ __forceinline__ void core_addsub(const Group<L> &g, uint32_t *lds, in
     uint32_t b2 = (rw != 0 || sticky) ? 1u : 0u;
     rw = 0u - rw - sticky;
     g.sub(T, X, Ys, b2);
-    uint32_t top = g.bcast(T[K - 1], L - 1);
+    uint32_t top = g.bcast_top(T[K - 1]);
     E = Ex;
     if (!(top >> 31)) {
       /* normalise the (n+1)-word value [T, rw]; more than one bit only when d == 1 (exact) */
@@ -695,9 +695,9 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
     uint64_t co_l = normalize_columns<KD>(cl, dl, 0);
     uint64_t co_h = normalize_columns<KD>(ch, dh, 0);
     /* multi-bit carries from the preceding chunk */
-    uint32_t pl_lo = g.from_prev((uint32_t)co_l), pl_hi = g.from_prev((uint32_t)(co_l >> 32));
-    uint32_t ph_lo = g.from_prev((uint32_t)co_h), ph_hi = g.from_prev((uint32_t)(co_h >> 32));
-    uint64_t top_l = ((uint64_t)g.bcast((uint32_t)(co_l >> 32), L - 1) << 32) | g.bcast((uint32_t)co_l, L - 1);
+    uint32_t pl_lo = g.from_prev_row0((uint32_t)co_l), pl_hi = g.from_prev_row0((uint32_t)(co_l >> 32));
+    uint32_t ph_lo = g.from_prev_row0((uint32_t)co_h), ph_hi = g.from_prev_row0((uint32_t)(co_h >> 32));
+    uint64_t top_l = ((uint64_t)g.bcast_top((uint32_t)(co_l >> 32)) << 32) | g.bcast_top((uint32_t)co_l);
     uint64_t in_l = ((uint64_t)pl_hi << 32) | pl_lo;
     uint64_t in_h = g.gl == 0 ? top_l : (((uint64_t)ph_hi << 32) | ph_lo);
     /* chunk carries are below 2^38: a column sum is below 2^64 */
@@ -733,13 +733,14 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
     constexpr int NB = mul_extract_nb(K, L), DL = mul_delta(K), NDG = NB + KD;
     constexpr int SMIN = MUL_RADIX * NB - 1 + DL * (2 * L - 1), OFF = SMIN / 32 * 32;
     constexpr int hd = topdig - ND;
-    full = (g.bcast(dh[hd % KD], hd / KD) >> topoff) & 1u;
+    full = (g.template bcast_lane<hd / KD>(dh[hd % KD]) >> topoff) & 1u;
     uint32_t Dg[NDG];
-    const int pred = (g.gl + L - 1) % L;
+    /* the predecessor lane's top high digits; lane 0 takes the low half's top digits, held
+     * by lane L-1 */
 #pragma unroll
     for (int m = 0; m < NB; m++) {
-      const uint32_t ex = g.gl == L - 1 ? dl[KD - NB + m] : dh[KD - NB + m];
-      Dg[m] = g.bcast(ex, pred);
+      const uint32_t fromtop = g.bcast_top(dl[KD - NB + m]);
+      Dg[m] = g.from_prev_row(dh[KD - NB + m], fromtop);
     }
 #pragma unroll
     for (int k = 0; k < KD; k++) Dg[NB + k] = dh[k];
@@ -767,7 +768,7 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
 #pragma unroll
     for (int i = 0; i < K; i++) H[i] = __builtin_amdgcn_alignbit(sel[i + 2], sel[i + 1], sh);
     /* the round word: the 32 bits below group lane 0's first mantissa bit */
-    rw = g.bcast(__builtin_amdgcn_alignbit(sel[1], sel[0], sh), 0);
+    rw = g.template bcast_lane<0>(__builtin_amdgcn_alignbit(sel[1], sel[0], sh));
   } else {
     /* digits to LDS: [0, ND) low, [ND, 2ND) high, then two zero words */
 #pragma unroll
