@@ -1209,6 +1209,44 @@ __device__ __forceinline__ void recode_digits_top(const Group<L> &g, uint32_t *l
   wsync();
 }
 
+/* recode_digits_top without LDS, for shapes whose lanes' digit windows drift by at most a
+ * word over the group (16 and 32 limbs per lane): lane gl's digits start E*(gl - L) + extra
+ * bits from its own word 0 (E = 27*KD - 32K, PAD = E*L), i.e. in the previous lane's top word,
+ * and end at most one bit into the next lane's word 0; those two words come through DPP
+ * (zero below and above the value).  The window starts c bits lower still where E = 1, so
+ * that its start stays below word 0 for every lane and extra; the digits then sit c bits
+ * into it. */
+template <int K, int L, int KD>
+__host__ __device__ constexpr bool recode_top_dpp_ok() {
+  constexpr int E = DIV_RADIX * KD - 32 * K, c = E >= 2 ? 0 : 2 - E;
+  return L >= 2 && L <= 32 && E >= 1 && E * L + c <= 32;
+}
+template <int K, int L, int KD>
+__device__ __forceinline__ void recode_digits_top_dpp(const Group<L> &g, const uint32_t (&X)[K], uint32_t (&D)[KD],
+                                                      int extra = 0) {
+  constexpr int E = DIV_RADIX * KD - 32 * K, c = E >= 2 ? 0 : 2 - E;
+  constexpr int NW = (DIV_RADIX * KD + c + 31) / 32;
+  constexpr uint32_t M = (1u << DIV_RADIX) - 1;
+  static_assert(recode_top_dpp_ok<K, L, KD>(), "digit window within the lane and its neighbours");
+  static_assert(NW <= K + 1, "the window ends in the next lane's word 0");
+  uint32_t w[K + 2];
+  w[0] = g.from_prev_row(X[K - 1], 0u);
+#pragma unroll
+  for (int i = 0; i < K; i++) w[i + 1] = X[i];
+  w[K + 1] = g.from_next_row0(X[0]);
+  /* window start, bits from the lane's word 0: E*(gl-L) + extra - c, in [-32, -1] */
+  const uint32_t sh = (uint32_t)(32 + E * (g.gl - L) + extra - c);
+  uint32_t win[NW + 1];
+#pragma unroll
+  for (int i = 0; i < NW; i++) win[i] = __builtin_amdgcn_alignbit(w[i + 1], w[i], sh);
+  win[NW] = 0;
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    const int j = (DIV_RADIX * k + c) >> 5, s2 = (DIV_RADIX * k + c) & 31;
+    D[k] = (s2 + DIV_RADIX <= 32 ? win[j] >> s2 : __builtin_amdgcn_alignbit(win[j + 1], win[j], (uint32_t)s2)) & M;
+  }
+}
+
 /* Exact normalisation, in place, of N signed columns per lane (lane gl holds positions
  * gl*N .. +N-1) into digits in [0, 2^27); returns the carry out of the top lane (the value's
  * top part): value = sum C_k beta^k + top * beta^(N*L).  Columns below 2^63 in magnitude. */
@@ -1266,8 +1304,13 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     for (int i = 0; i < K; i++) R0[i] = A[i];
   }
   uint32_t dd[KD], rd[KD];
-  recode_digits_top<K, L, KD>(g, lds, B, dd);
-  recode_digits_top<K, L, KD>(g, lds, R0, rd);
+  if constexpr (recode_top_dpp_ok<K, L, KD>()) {
+    recode_digits_top_dpp<K, L, KD>(g, B, dd);
+    recode_digits_top_dpp<K, L, KD>(g, R0, rd);
+  } else {
+    recode_digits_top<K, L, KD>(g, lds, B, dd);
+    recode_digits_top<K, L, KD>(g, lds, R0, rd);
+  }
   int32_t nd[KD];
   int64_t C[KD];
 #pragma unroll
@@ -1541,7 +1584,10 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   e += o;
   const int64_t er = e / 2;
   uint32_t ad[KD];
-  recode_digits_top<K, L, KD>(g, lds, A, ad, o);
+  if constexpr (recode_top_dpp_ok<K, L, KD>())
+    recode_digits_top_dpp<K, L, KD>(g, A, ad, o);
+  else
+    recode_digits_top<K, L, KD>(g, lds, A, ad, o);
   constexpr int KH = (KD + 1) / 2, HF = KH * L, P2 = F - HF;
   constexpr bool kRen = F > 64;
   static_assert(KH >= 2, "the first trip holds the two special steps");
