@@ -5,7 +5,8 @@ value, K*L = precision/32).  For every (op, precision) this times every shape on
 same synthetic operands (fpa_workload_generate kind 2: e ~ U[-1024, 1024], 1% zeros, 5%
 near-cancellation pairs), checks that every shape returns bit-identical results, and
 writes the fastest shape as one "op precision K L" line (op 6 = the dot-chain kernel,
-timed on chains of U[8, 512] elements).  It then times every shape again on small
+timed on 16,384 chains of U[8, 512] elements, the size of the 2048-bit dot workload, with
+shorter chains above 2048 bits; its tiers count chains, 512 and 2,048 of them).  It then times every shape again on small
 launches (2,048 to 16,384 values by default, the sizes of the segments of a 2^18-op mixed
 batch or of a served request): such a launch cannot fill the chip and the shape with the
 shortest per-value latency wins, a different one at different sizes.  Where a size's
@@ -61,18 +62,21 @@ def time_variant(op, precision, args, variant, target_ms=20.0):
     return float(ms.value)
 
 
+DOT_CHAINS = 16384
+DOT_SMALL_CHAINS = (512, 2048)
+
+
 def tune_dot(precision, count=None, seed=1, log=None):
-    """Dot chains (fpa_dot_plan_run): chains of U[8, 512] elements, `count` elements in
-    all (an eighth of tune_count's mul figure by default)."""
-    count = count or max(tune_count("mul", precision) // 8, 4096)
+    """Dot chains (fpa_dot_plan_run): `count` chains (default DOT_CHAINS) of U[8, m]
+    elements, m = 512 up to 2048 bits and 512*2048/precision above (at least 16), so an
+    operand array stays near 1 GB.  A dot kernel keeps every chain of a wave in registers
+    for the wave's whole life, so how many chains there are (how many waves, and whether
+    they are all resident at once) decides the shape as much as the precision does: the
+    chain count is what the plan's shape lookup (and its tiers) go by."""
+    chains = count or DOT_CHAINS
+    m = 512 if precision <= 2048 else max(16, 512 * 2048 // precision)
     rng = np.random.default_rng(seed + precision)
-    lengths = []
-    while sum(lengths) < count:
-        lengths.append(int(rng.integers(8, 513)))
-    lengths[-1] -= sum(lengths) - count
-    if lengths[-1] <= 0:
-        lengths.pop()
-    lengths = np.array(lengths, np.uint32)
+    lengths = rng.integers(8, m + 1, chains).astype(np.uint32)
     total = int(lengths.sum())
     xe, xl, ye, yl = generate(1, seed + precision, "mul", precision, total)
     X, Y, R = FpaArray(total, precision), FpaArray(total, precision), FpaArray(len(lengths), precision)
@@ -100,7 +104,7 @@ def tune_dot(precision, count=None, seed=1, log=None):
         if log:
             parts = ", ".join(f"{k}x{l} {ms:.4g} ms{'*' if (k, l) == best[:2] else ''}" for k, l, ms in timings)
             log(f"dot {precision} ({total} elements, {len(lengths)} chains): {parts}")
-        return best[:2], timings, total
+        return best[:2], timings, chains
     finally:
         set_variant("dot", precision, keep)
         for x in (X, Y, R):
@@ -172,10 +176,11 @@ def tune(ops=TUNED_OPS, precisions=PRECISIONS, log=None, small_counts=SMALL_COUN
             best, timings, count = tune_one(op, p, log=log)
             out[(op, p)] = (best, timings, count)
             set_variant(op, p, variants(op, p).index(best))
-            if not small_counts or op == "dot":
+            if not small_counts:
                 continue
-            bounds = tier_bounds(small_counts)
-            tiers = [tune_one(op, p, count=c, log=log) + (b,) for c, b in zip(sorted(small_counts), bounds)]
+            sizes = DOT_SMALL_CHAINS if op == "dot" else small_counts
+            bounds = tier_bounds(sizes)
+            tiers = [tune_one(op, p, count=c, log=log) + (b,) for c, b in zip(sorted(sizes), bounds)]
             differ = [i for i, t in enumerate(tiers) if t[0] != best]
             if differ:
                 small[(op, p)] = tiers[:differ[-1] + 1]
@@ -190,11 +195,12 @@ def write_table(path, table, small=None):
              "# op precision K L [max_count]   (per-shape mean launch time; * = chosen)"]
     for (op, p), (best, timings, count) in sorted(table.items(), key=lambda kv: (KERNELS[kv[0][0]], kv[0][1])):
         parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == best else "") for k, l, ms in timings)
-        lines.append(f"# {op} {p}, {count} {'elements' if op == 'dot' else 'values'}: {parts}")
+        lines.append(f"# {op} {p}, {count} {'chains' if op == 'dot' else 'values'}: {parts}")
         lines.append(f"{KERNELS[op]} {p} {best[0]} {best[1]}")
         for sbest, stimings, scount, bound in small.get((op, p), []):
             parts = ", ".join(f"{k}x{l} {ms:.4g} ms" + ("*" if (k, l) == sbest else "") for k, l, ms in stimings)
-            lines.append(f"# {op} {p}, small launches of {scount} values (tier up to {bound}): {parts}")
+            unit = "chains" if op == "dot" else "values"
+            lines.append(f"# {op} {p}, small launches of {scount} {unit} (tier up to {bound}): {parts}")
             lines.append(f"{KERNELS[op]} {p} {sbest[0]} {sbest[1]} {bound}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -108,6 +108,19 @@ template <> struct MulSched<32, 27> {
   __device__ static uint32_t lane(int gl) { return k_mul_sched_32_27[gl]; }
 };
 
+/* L = 32, F0 = 23: 16 + 10 rounds per lane (832 slots for 748 full pairs), 23 guard pairs whole, 0 corners; 55 lane-chunk collisions over 28 emissions */
+__constant__ uint32_t k_mul_sched_32_23[32] = {
+  0x00068a80u, 0x00070aa0u, 0x00078ac0u, 0x00080ae0u, 0x00088b00u, 0x00090b20u, 0x00098b40u, 0x0003b360u,
+  0x00043380u, 0x0004b3a0u, 0x000533c0u, 0x0006b3e0u, 0x00073166u, 0x0007b186u, 0x00083030u, 0x0008b050u,
+  0x00093070u, 0x0009b090u, 0x000058b0u, 0x0003d8d0u, 0x00045a90u, 0x0004dab0u, 0x00055ad0u, 0x0005daf0u,
+  0x00065b10u, 0x0006db30u, 0x00075b50u, 0x0007db70u, 0x00085b90u, 0x0008dbb0u, 0x00095bd0u, 0x0009dbf0u,
+};
+template <> struct MulSched<32, 23> {
+  static constexpr bool ok = true;
+  static constexpr int R1 = 16, R2 = 10, NCORNER = 0;
+  __device__ static uint32_t lane(int gl) { return k_mul_sched_32_23[gl]; }
+};
+
 }  // namespace fpa
 
 #endif

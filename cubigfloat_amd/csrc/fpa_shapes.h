@@ -11,7 +11,7 @@
   X(8, 8, 1)     X(8, 4, 2)     X(8, 2, 4)                   \
   X(16, 8, 2)    X(16, 16, 1)   X(16, 4, 4)                  \
   X(32, 8, 4)    X(32, 16, 2)   X(32, 4, 8)                  \
-  X(64, 8, 8)    X(64, 16, 4)   X(64, 4, 16)  X(64, 32, 2)   \
+  X(64, 8, 8)    X(64, 16, 4)   X(64, 4, 16)  X(64, 32, 2)  X(64, 2, 32)  \
   X(96, 12, 8)   X(96, 6, 16)   X(96, 24, 4)                 \
   X(128, 16, 8)  X(128, 8, 16)  X(128, 32, 4) X(128, 4, 32)  \
   X(160, 10, 16) X(160, 20, 8)  X(160, 5, 32)                \

@@ -178,7 +178,7 @@ def check(L, F0, R1, R2, table):
 
 # (L, F0) pairs the compiled shapes use: F0 = L - 1, and for L = 32 min(31, TMAX // KD)
 # with TMAX = (p - 133) // 27 (MulShort in fpa_ops.h)
-PAIRS = ((4, 3), (8, 7), (16, 15), (32, 31), (32, 30), (32, 29), (32, 28), (32, 27))
+PAIRS = ((4, 3), (8, 7), (16, 15), (32, 31), (32, 30), (32, 29), (32, 28), (32, 27), (32, 23))
 
 
 def main():
