@@ -466,12 +466,15 @@ __device__ __forceinline__ void recode_digits_dpp(const Group<L> &g, const uint3
   }
 }
 
-/* local radix-2^27 normalisation of KD column sums with carry-in c; returns the carry out */
-template <int KD>
+/* local radix-2^27 normalisation of KD column sums with carry-in c; returns the carry out.
+ * S > 0: the columns below S are zero and c is zero (their digits are zero). */
+template <int KD, int S = 0>
 __device__ __forceinline__ uint64_t normalize_columns(const uint64_t (&col)[KD], uint32_t (&dg)[KD], uint64_t c) {
   constexpr uint32_t M = (1u << MUL_RADIX) - 1;
 #pragma unroll
-  for (int k = 0; k < KD; k++) {
+  for (int k = 0; k < S; k++) dg[k] = 0;
+#pragma unroll
+  for (int k = S; k < KD; k++) {
     uint64_t v = col[k] + c;
     dg[k] = (uint32_t)v & M;
     c = v >> MUL_RADIX;
@@ -520,7 +523,7 @@ __device__ __forceinline__ uint32_t add_carry_digits_wave(uint32_t (&dg)[KD], ui
   return c32;
 }
 
-template <int K, int L, bool kShort>
+template <int K, int L, bool kShort, int LOWC0 = -1>
 __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
                                            const uint64_t (&cl)[mul_digits_per_lane(K)],
                                            const uint64_t (&ch)[mul_digits_per_lane(K)], int mode, uint32_t ea,
@@ -677,8 +680,9 @@ __device__ __forceinline__ void core_mul(const Group<L> &g, uint32_t *lds, int g
  * omitted digit products all lie in columns below T, mul_short); the rounding is then
  * decided by Q alone, with sticky 1, unless the 64 bits of Q just below the round word
  * are all zeros or all ones: amb reports that case (the same for every lane of the group),
- * and the caller recomputes the product in full. */
-template <int K, int L, bool kShort>
+ * and the caller recomputes the product in full.  LOWC0 >= 0: only the top lane holds a
+ * low chunk, whose columns below LOWC0 are zero (no carry enters a low chunk). */
+template <int K, int L, bool kShort, int LOWC0>
 __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
                                            const uint64_t (&cl)[mul_digits_per_lane(K)],
                                            const uint64_t (&ch)[mul_digits_per_lane(K)], int mode, uint32_t ea,
@@ -692,16 +696,18 @@ __device__ __forceinline__ void mul_finish(const Group<L> &g, uint32_t *lds,
   uint32_t dl[KD], dh[KD];
   {
     wsync();   /* every lane has read its columns before the digits overwrite them */
-    uint64_t co_l = normalize_columns<KD>(cl, dl, 0);
+    uint64_t co_l = normalize_columns<KD, (LOWC0 > 0 ? LOWC0 : 0)>(cl, dl, 0);
     uint64_t co_h = normalize_columns<KD>(ch, dh, 0);
     /* multi-bit carries from the preceding chunk */
-    uint32_t pl_lo = g.from_prev_row0((uint32_t)co_l), pl_hi = g.from_prev_row0((uint32_t)(co_l >> 32));
     uint32_t ph_lo = g.from_prev_row0((uint32_t)co_h), ph_hi = g.from_prev_row0((uint32_t)(co_h >> 32));
     uint64_t top_l = ((uint64_t)g.bcast_top((uint32_t)(co_l >> 32)) << 32) | g.bcast_top((uint32_t)co_l);
-    uint64_t in_l = ((uint64_t)pl_hi << 32) | pl_lo;
     uint64_t in_h = g.gl == 0 ? top_l : (((uint64_t)ph_hi << 32) | ph_lo);
     /* chunk carries are below 2^38: a column sum is below 2^64 */
-    uint32_t ol = add_carry_digits_wave<KD>(dl, in_l);
+    uint32_t ol = 0;
+    if constexpr (LOWC0 < 0) {
+      uint32_t pl_lo = g.from_prev_row0((uint32_t)co_l), pl_hi = g.from_prev_row0((uint32_t)(co_l >> 32));
+      ol = add_carry_digits_wave<KD>(dl, ((uint64_t)pl_hi << 32) | pl_lo);
+    }
     uint32_t oh = add_carry_digits_wave<KD>(dh, in_h);
     /* 1-bit carries along the chain: bit t = low chunk t, bit L+t = high chunk t */
     bool pl = true, ph = true;
@@ -1078,7 +1084,8 @@ __device__ __forceinline__ void core_mul_short(const Group<L> &g, uint32_t *lds,
     }
   }
   bool amb;
-  mul_finish<K, L, true>(g, lds, cl, ch, mode, ea, eb, re, R, amb);
+  /* with F0 = L-1 only the top lane holds a low chunk, chunk L-1, from column C0 on */
+  mul_finish<K, L, true, S::F0 == L - 1 ? C0 : -1>(g, lds, cl, ch, mode, ea, eb, re, R, amb);
   if constexpr (!kLockstep) {
     /* this group is undecided: it forms its product in full */
     if (__builtin_expect(amb, 0)) core_mul<K, L, false>(g, lds, group_stride, mode, ea, Aw, eb, Bw, re, R);
