@@ -1318,13 +1318,19 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     recode_digits_top<K, L, KD>(g, lds, B, dd);
     recode_digits_top<K, L, KD>(g, lds, R0, rd);
   }
-  int32_t nd[KD];
-  int64_t C[KD];
-#pragma unroll
-  for (int i = 0; i < KD; i++) { nd[i] = -(int32_t)dd[i]; C[i] = (int64_t)rd[i]; }
   /* divisor's top two digits (meaningful in the top lane); inv32 = 2^32 / Dt scales the
    * step estimate, which works on V / 2^32 */
   const double inv = 1.0 / __builtin_fma((double)dd[KD - 1], (double)BETA, (double)dd[KD - 2]);
+  int32_t nd[KD];
+  int64_t C[KD];
+#pragma unroll
+  for (int i = 0; i < KD; i++) {
+    nd[i] = -(int32_t)dd[i];
+    /* opaque, so the exact end path's -nd[i] is not taken from dd[i], which would keep the
+     * divisor's digits live through the digit loop beside their negations (KD registers) */
+    asm volatile("" : "+v"(nd[i]));
+    C[i] = (int64_t)rd[i];
+  }
   const double inv32 = inv * 0x1p32;
   const bool top = g.gl == L - 1;
   int32_t *qd = reinterpret_cast<int32_t *>(lds);   /* quotient digit of step j at S-1-j */
@@ -1354,16 +1360,16 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     if constexpr (ND > 255) {
       /* one carry level every 128 steps keeps the columns far below 2^63 */
       if ((j & 127) == 127) {
-        int64_t car[KD];
-#pragma unroll
-        for (int i = 0; i < KD; i++) car[i] = C[i] >> DIV_RADIX;
-        const uint32_t clo = g.from_prev_row0((uint32_t)car[KD - 1]);
-        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KD - 1] >> 32));
+        /* in place from the top down (column i takes column i-1's carry before that
+         * column is reduced), so no second array of columns is live */
+        const int64_t ctop = C[KD - 1] >> DIV_RADIX;
+        const uint32_t clo = g.from_prev_row0((uint32_t)ctop);
+        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)ctop >> 32));
         const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
 #pragma unroll
         for (int i = KD - 1; i >= 0; i--) {
           const int64_t keep = (top && i == KD - 1) ? C[i] : (C[i] & (int64_t)MASK);
-          C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
+          C[i] = keep + (i > 0 ? (C[i - 1] >> DIV_RADIX) : cbelow);
         }
       }
     }
