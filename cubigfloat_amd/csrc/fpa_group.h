@@ -357,6 +357,16 @@ struct Group {
     else return from_prev_row(x, 0u);
   }
 
+  /* from_prev_row0 with the zero in group lane 0 from a lane mask nz0 (all ones except in
+   * group lane 0) that the caller keeps opaque: the AND takes the DPP move as its first
+   * source (one v_and_b32 with a DPP operand instead of a move and a select) */
+  __device__ __forceinline__ uint32_t from_prev_row_and(uint32_t x, uint32_t nz0) const {
+    if constexpr (L == 16) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x111, 0xF, 0xF, true);
+    else if constexpr (L >= 2 && L <= 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x111, 0xF, 0xF, true) & nz0;
+    else if constexpr (L == 32) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x138, 0xF, 0xF, true) & nz0;
+    else return from_prev(x, 0u);
+  }
+
   /* x of group lane gl+1, 'fill' in the group's top lane: DPP row_shl:1 for groups inside
    * one 16-lane row (a group's top lane reads the next group's lane 0, or zero at the row's
    * end, and takes 'fill' instead), ds_bpermute for larger groups */

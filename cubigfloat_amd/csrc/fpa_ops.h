@@ -1176,6 +1176,43 @@ __device__ __forceinline__ double i64_to_f64(int64_t x) {
   return __builtin_fma((double)(int32_t)((uint64_t)x >> 32), 4294967296.0, (double)(uint32_t)x);
 }
 
+/* A lazy frame's digit estimate floor(x), x = V * inv32 (|x| < 2^31), as x - 1/2 rounded to
+ * nearest by adding 1.5 * 2^52, whose low word is then the integer: one add in place of a
+ * floor and a conversion.  The fma's rounding (below 2^-23) can move the result by one where
+ * x lies that close to an integer: the remainder then ends at most about 2^-22 of the divisor
+ * outside [0, D), well inside the (-D, 2D) that the digit steps keep (the next digit comes
+ * out one higher or lower), and the end correction treats such a remainder as unsure. */
+__device__ __forceinline__ int32_t lazy_digit(double V, double inv32) {
+  const double y = __builtin_fma(V, inv32, -0.5) + 0x1.8p52;
+  return (int32_t)(uint32_t)__builtin_bit_cast(uint64_t, y);
+}
+
+/* The digit estimate from a lazy frame's top two columns c1 (top) and c0: floor(V * 2^32 / Dt)
+ * for V = c1 * 2^-5 + floor(c0 / 2^32), i.e. floor(W * invs) for the integer W = 32 V =
+ * c1 + 32 floor(c0 / 2^32) (one v_mad_i64_i32) and invs = 2^27 / Dt */
+__device__ __forceinline__ int32_t lazy_top_digit(int64_t c1, int64_t c0, double invs) {
+  int32_t k32;   /* 32 in an SGPR the compiler cannot see through: it would shift and add */
+  asm("s_mov_b32 %0, 32" : "=s"(k32));
+  const int64_t w = (int64_t)(int32_t)((uint64_t)c0 >> 32) * (int64_t)k32 + c1;
+  return lazy_digit(i64_to_f64(w), invs);
+}
+
+/* U <- U * beta on a lazy frame of KC columns per lane: the columns move up one slot, the one
+ * leaving a lane enters the next lane's bottom slot (zero in group lane 0), the one leaving
+ * the top lane folds into the new top column.  nz0 (all ones except in group lane 0) is a
+ * lane mask the caller keeps opaque, so that it stays an AND: the move from the previous
+ * lane is then one VALU per word (the AND reads the DPP move directly). */
+template <int KC, int L>
+__device__ __forceinline__ void lazy_shift_up(const Group<L> &g, int64_t (&C)[KC], uint32_t nz0, bool top) {
+  const uint64_t so = (uint64_t)C[KC - 1];
+  const uint32_t lo = g.from_prev_row_and((uint32_t)so, nz0);
+  const uint32_t hi = g.from_prev_row_and((uint32_t)(so >> 32), nz0);
+#pragma unroll
+  for (int i = KC - 1; i > 0; i--) C[i] = C[i - 1];
+  C[0] = (int64_t)(((uint64_t)hi << 32) | lo);
+  C[KC - 1] = (int64_t)((uint64_t)C[KC - 1] + (top ? so << DIV_RADIX : 0u));
+}
+
 /* Digits of X (n = K*L words) in radix 2^27 aligned to the top of KD*L digits: digit k of
  * the value = bits [27k - PAD + extra, +27) of X, PAD = 27*KD*L - 32n (bits below X's bit
  * 0 read as zero).  lds: ZW + n + 2 words, ZW = ceil(PAD / 32). */
@@ -1318,8 +1355,8 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     recode_digits_top<K, L, KD>(g, lds, B, dd);
     recode_digits_top<K, L, KD>(g, lds, R0, rd);
   }
-  /* divisor's top two digits (meaningful in the top lane); inv32 = 2^32 / Dt scales the
-   * step estimate, which works on V / 2^32 */
+  /* divisor's top two digits (meaningful in the top lane); invs = 2^27 / Dt scales the
+   * step estimate */
   const double inv = 1.0 / __builtin_fma((double)dd[KD - 1], (double)BETA, (double)dd[KD - 2]);
   int32_t nd[KD];
   int64_t C[KD];
@@ -1331,25 +1368,17 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     asm volatile("" : "+v"(nd[i]));
     C[i] = (int64_t)rd[i];
   }
-  const double inv32 = inv * 0x1p32;
+  const double invs = inv * 0x1p27;
   const bool top = g.gl == L - 1;
+  uint32_t nz0 = g.gl == 0 ? 0u : ~0u;
+  asm volatile("" : "+v"(nz0));
   int32_t *qd = reinterpret_cast<int32_t *>(lds);   /* quotient digit of step j at S-1-j */
 
   auto step = [&](int j) __attribute__((always_inline)) {
-    /* R <- R * beta: columns move up; the column leaving a lane enters the next lane's
-     * bottom; the one leaving the top lane folds into the new top column */
-    const int64_t so = C[KD - 1];
-    const uint32_t inlo = g.from_prev_row0((uint32_t)so), inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
-#pragma unroll
-    for (int i = KD - 1; i > 0; i--) C[i] = C[i - 1];
-    C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
-    const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
-    C[KD - 1] = (int64_t)((uint64_t)C[KD - 1] + fold);
-    /* digit estimate from the top two columns (top lane), broadcast: V / 2^32 from
-     * c1 * 2^-5 + floor(c0 / 2^32) */
-    const double V = __builtin_fma(i64_to_f64(C[KD - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KD - 2] >> 32));
-    int32_t q = (int32_t)__builtin_floor(V * inv32);
-    q = (int32_t)g.bcast_top((uint32_t)q);
+    /* R <- R * beta */
+    lazy_shift_up<KD, L>(g, C, nz0, top);
+    /* digit estimate from the top two columns (top lane), broadcast */
+    const int32_t q = (int32_t)g.bcast_top((uint32_t)lazy_top_digit(C[KD - 1], C[KD - 2], invs));
 #pragma unroll
     for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nd[i] + C[i];
     qd[S - 1 - j] = q;
@@ -1380,7 +1409,7 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
   repeat<S % KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
 
   /* final remainder R = sum C_k beta^k lies in (-D, 2D); its estimate t ~ R/D */
-  const double t = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2])) * inv;
+  const double t = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2])) * (invs * 0x1p-27);
   const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
   uint32_t rem_nz = 1;
   int32_t delta = 0;
@@ -1511,31 +1540,22 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  * the owner lane's slot idx of T (zero so far) becomes 2q. */
 template <int KC, int L, bool kFill, bool kRenorm>
 __device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int k,
-                                               int idx, uint32_t ownm, int kind, bool top, uint32_t fill,
-                                               int32_t &q0, double &inv32) {
+                                               int idx, uint32_t ownm, int kind, bool top, uint32_t nz0,
+                                               uint32_t fill, int32_t &q0, double &invs) {
   constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
   constexpr double BETA_D = 134217728.0;   /* 2^27 */
   /* U <- U * beta */
-  const int64_t so = C[KC - 1];
-  uint32_t inlo = g.from_prev_row0((uint32_t)so);
-  const uint32_t inhi = g.from_prev_row0((uint32_t)((uint64_t)so >> 32));
-  if constexpr (kFill) inlo |= fill;
-#pragma unroll
-  for (int i = KC - 1; i > 0; i--) C[i] = C[i - 1];
-  C[0] = (int64_t)(((uint64_t)inhi << 32) | inlo);
-  const uint64_t fold = top ? ((uint64_t)so << DIV_RADIX) : 0u;
-  C[KC - 1] = (int64_t)((uint64_t)C[KC - 1] + fold);
+  lazy_shift_up<KC, L>(g, C, nz0, top);
+  if constexpr (kFill) C[0] = (int64_t)((uint64_t)C[0] | fill);
   int32_t q;
   if (kind == 0) {
     q = (int32_t)fpa_isqrt64((uint64_t)C[KC - 1]);
-  } else {
+  } else if (kind == 1) {
     const double Vs = __builtin_fma(i64_to_f64(C[KC - 1]), 0x1p-5, (double)(int32_t)((uint64_t)C[KC - 2] >> 32));
-    if (kind == 1) {
-      const double a = (double)q0 * BETA_D, V = Vs * 0x1p32;
-      q = (int32_t)__builtin_floor(V / (a + __builtin_sqrt(__builtin_fma(a, a, V))));
-    } else {
-      q = (int32_t)__builtin_floor(Vs * inv32);
-    }
+    const double a = (double)q0 * BETA_D, V = Vs * 0x1p32;
+    q = (int32_t)__builtin_floor(V / (a + __builtin_sqrt(__builtin_fma(a, a, V))));
+  } else {
+    q = lazy_top_digit(C[KC - 1], C[KC - 2], invs);
   }
   q = (int32_t)g.bcast_top((uint32_t)q);
   /* slot idx is still zero in the owner lane (ownm all ones; the lanes above it hold
@@ -1553,7 +1573,7 @@ __device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[K
   if (kind == 0) q0 = q;
   if (kind == 1) {
     const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
-    inv32 = 0x1p32 / tt;
+    invs = 0x1p27 / tt;
   }
   /* keep the scheduler from running later steps' digit chains ahead of this step's
    * multiply-adds (that holds several steps' columns live and costs occupancy) */
@@ -1605,7 +1625,9 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   constexpr bool kRen = F > 64;
   static_assert(KH >= 2, "the first trip holds the two special steps");
   const bool top = g.gl == L - 1;
-  double inv32 = 0.0;                      /* 2^32 / (T_{F-1} beta + T_{F-2}), from step 2 on */
+  uint32_t nz0 = g.gl == 0 ? 0u : ~0u;     /* opaque: the column moves stay ANDs */
+  asm volatile("" : "+v"(nz0));
+  double invs = 0.0;                       /* 2^27 / (T_{F-1} beta + T_{F-2}), from step 2 on */
   int32_t q0 = 0;
 
   /* Phase 1, steps 0 .. HF-1: T fits the frame's top HF slots, and U's slots below them only
@@ -1641,15 +1663,15 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     uint32_t ownm = top ? ~0u : 0u;
     asm volatile("" : "+v"(ownm));
     repeat<KH>([&](int u) __attribute__((always_inline)) {
-      sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, fl[u],
-                                         q0, inv32);
+      sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0,
+                                         fl[u], q0, invs);
     });
     for (int tr = 1; tr < L; tr++) {
       fills(tr);
       ownm = g.gl == L - 1 - tr ? ~0u : 0u;
       asm volatile("" : "+v"(ownm));
       repeat<KH>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, fl[u], q0, inv32);
+        sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
       });
     }
   }
@@ -1688,20 +1710,20 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     uint32_t ownm = g.gl == LT ? ~0u : 0u;
     asm volatile("" : "+v"(ownm));
     repeat<RT>([&](int u) __attribute__((always_inline)) {
-      sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, 0u, q0, inv32);
+      sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
     });
     for (int tr = LT - 1; tr >= 0; tr--) {
       ownm = g.gl == tr ? ~0u : 0u;
       asm volatile("" : "+v"(ownm));
       const int k0 = HF + RT + (LT - 1 - tr) * KD;
       repeat<KD>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, 0u, q0, inv32);
+        sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
       });
     }
   }
 
   /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
-  const double r = __builtin_fma(i64_to_f64(C[KD - 1]), BETA_D, i64_to_f64(C[KD - 2])) * inv32 * 0x1p-32;
+  const double r = __builtin_fma(i64_to_f64(C[KD - 1]), BETA_D, i64_to_f64(C[KD - 2])) * invs * 0x1p-27;
   const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
   uint32_t rem_nz = 1;
   int32_t delta = 0;
