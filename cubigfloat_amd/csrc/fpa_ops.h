@@ -1155,13 +1155,25 @@ __host__ __device__ constexpr int div_lds_words(int K, int L) {
                                                                    : div_steps(K * L) + 3;
 }
 
+/* Columns per lane of the short sqrt's frame: the top FB = KB*L >= (F + 2) / 2 slots of the
+ * F = KD*L root slots (ceil) */
+__host__ __device__ constexpr int sqrt_short_cols(int KD, int L) { return (KD * L + 2 + 2 * L - 1) / (2 * L); }
+template <int K, int L>
+__host__ __device__ constexpr bool sqrt_short_ok() {
+  constexpr int KD = div_digits_per_lane(K), F = KD * L, KB = sqrt_short_cols(KD, L), B0 = F - KB * L;
+  return KB >= 2 && KB < KD && B0 >= 1 && 2 * B0 <= F - 2;
+}
+
 /* LDS words per value of the lazy sqrt: recoding stage, the radicand digits of the half-frame
- * phase (with L zeros below and KH above), or the root digits (+ three zeros) */
+ * phase (with L zeros below and KH above) and of the short frame (2L + 2 zeros below, KB
+ * above), or the root digits (+ three zeros) */
 template <int K, int L>
 __host__ __device__ constexpr int sqrt_lazy_lds_words() {
-  constexpr int recode = div_pad_words(K, L) + K * L + 2, F = div_digits_per_lane(K) * L;
-  constexpr int half = L + F + (div_digits_per_lane(K) + 1) / 2, root = F + 3;
-  return recode > half ? (recode > root ? recode : root) : (half > root ? half : root);
+  constexpr int KD = div_digits_per_lane(K), recode = div_pad_words(K, L) + K * L + 2, F = KD * L;
+  constexpr int half = L + F + (KD + 1) / 2, root = F + 3;
+  constexpr int shrt = sqrt_short_ok<K, L>() ? 2 * L + 2 + F + sqrt_short_cols(KD, L) : 0;
+  constexpr int m1 = recode > half ? recode : half, m2 = root > shrt ? root : shrt;
+  return m1 > m2 ? m1 : m2;
 }
 
 /* x as held by group lane src, 64 bits */
@@ -1211,6 +1223,25 @@ __device__ __forceinline__ void lazy_shift_up(const Group<L> &g, int64_t (&C)[KC
   for (int i = KC - 1; i > 0; i--) C[i] = C[i - 1];
   C[0] = (int64_t)(((uint64_t)hi << 32) | lo);
   C[KC - 1] = (int64_t)((uint64_t)C[KC - 1] + (top ? so << DIV_RADIX : 0u));
+}
+
+/* One carry level over a lazy frame, in place from the top down (column i takes column
+ * i-1's carry before that column is reduced): every column but the top lane's top one keeps
+ * its low 27 bits and takes the carry of the column below (zero into group lane 0's bottom).
+ * The value is unchanged.  The steps run it between trips, never inside one, so that its
+ * code is not repeated at every unrolled step. */
+template <int KC, int L>
+__device__ __forceinline__ void lazy_renorm(const Group<L> &g, int64_t (&C)[KC], bool top) {
+  constexpr int64_t MASK = ((int64_t)1 << DIV_RADIX) - 1;
+  const int64_t ctop = C[KC - 1] >> DIV_RADIX;
+  const uint32_t clo = g.from_prev_row0((uint32_t)ctop);
+  const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)ctop >> 32));
+  const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
+#pragma unroll
+  for (int i = KC - 1; i >= 0; i--) {
+    const int64_t keep = (top && i == KC - 1) ? C[i] : (C[i] & MASK);
+    C[i] = keep + (i > 0 ? (C[i - 1] >> DIV_RADIX) : cbelow);
+  }
 }
 
 /* Digits of X (n = K*L words) in radix 2^27 aligned to the top of KD*L digits: digit k of
@@ -1386,26 +1417,17 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
      * multiply-adds: that lookahead holds several steps' columns live at once and costs
      * occupancy, while other waves hide the chain's latency anyway */
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (ND > 255) {
-      /* one carry level every 128 steps keeps the columns far below 2^63 */
-      if ((j & 127) == 127) {
-        /* in place from the top down (column i takes column i-1's carry before that
-         * column is reduced), so no second array of columns is live */
-        const int64_t ctop = C[KD - 1] >> DIV_RADIX;
-        const uint32_t clo = g.from_prev_row0((uint32_t)ctop);
-        const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)ctop >> 32));
-        const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
-#pragma unroll
-        for (int i = KD - 1; i >= 0; i--) {
-          const int64_t keep = (top && i == KD - 1) ? C[i] : (C[i] & (int64_t)MASK);
-          C[i] = keep + (i > 0 ? (C[i - 1] >> DIV_RADIX) : cbelow);
-        }
-      }
-    }
   };
   wsync();   /* the recoding reads of lds are done before the digits overwrite them */
   int j0 = 0;
-  for (; j0 + KD <= S; j0 += KD) repeat<KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+  for (; j0 + KD <= S; j0 += KD) {
+    if constexpr (ND > 255) {
+      /* one carry level at most every 128 steps keeps the columns far below 2^63 */
+      constexpr int RT = 128 / KD;
+      if (j0 > 0 && (j0 / KD) % RT == 0) lazy_renorm<KD, L>(g, C, top);
+    }
+    repeat<KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+  }
   repeat<S % KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
 
   /* final remainder R = sum C_k beta^k lies in (-D, 2D); its estimate t ~ R/D */
@@ -1492,15 +1514,26 @@ __device__ __forceinline__ void op_div(const fpa_arguments &a, const Slot &s, co
 }
 
 /* -------------------------------- sqrt -------------------------------- */
-template <int K, int L>
-__device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
+template <int K, int L, bool kShort>
+__device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
                                                const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]);
 
 template <int K, int L>
 __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, const Group<L> &g, uint32_t *lds) {
   uint32_t A[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
-  core_sqrt_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  if constexpr (sqrt_short_ok<K, L>()) {
+    if (!core_sqrt_lazy<K, L, true>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R)) {
+      /* the wave's short frames left a root undecided: the whole wave runs the full frame,
+       * from the operand reloaded (the clobber keeps the compiler from holding the first
+       * load's registers through the short run instead) */
+      asm volatile("" ::: "memory");
+      load_value<K, L>(g, a.a_limbs, s, A);
+      core_sqrt_lazy<K, L, false>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+    }
+  } else {
+    core_sqrt_lazy<K, L, false>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
+  }
   store_result(a, s, g.gl, re, R);
 }
 
@@ -1534,12 +1567,12 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  * read from them.  The reference's root (fpa_warp_sqrt.h:195-254) runs 32-bit digits
  * with exact remainders instead.
  */
-/* Digit step k of the lazy sqrt on a frame of KC columns per lane (the whole frame, or the
- * top half of it in the first phase).  U <- U * beta (kFill: 'fill', nonzero only in the
- * group's lane 0, enters the bottom slot), digit q from the top two columns, U -= q * T,
- * the owner lane's slot idx of T (zero so far) becomes 2q. */
-template <int KC, int L, bool kFill, bool kRenorm>
-__device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int k,
+/* Digit step k of the lazy sqrt on a frame of KC columns per lane (the whole frame, or its
+ * top part).  U <- U * beta (kFill: 'fill', nonzero only in the group's lane 0, enters the
+ * bottom slot), digit q from the top two columns, U -= q * T, the owner lane's slot idx of
+ * T (zero so far) becomes 2q (kOwn: the step's slot lies in the frame).  Returns q. */
+template <int KC, int L, bool kFill, bool kOwn = true>
+__device__ __forceinline__ int32_t sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int k,
                                                int idx, uint32_t ownm, int kind, bool top, uint32_t nz0,
                                                uint32_t fill, int32_t &q0, double &invs) {
   constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
@@ -1562,14 +1595,18 @@ __device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[K
    * earlier digits there): the owner sets it to -q for the multiply-adds, so they subtract
    * q^2 there too, then adds -q again for -2q (no branch on the owner lane) */
   const uint32_t t = (uint32_t)(-q) & ownm;
+  if constexpr (kOwn) {
 #pragma unroll
-  for (int i = 0; i < KC; i++)
-    if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
+    for (int i = 0; i < KC; i++)
+      if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
+  }
 #pragma unroll
   for (int i = 0; i < KC; i++) C[i] = (int64_t)q * nT[i] + C[i];
+  if constexpr (kOwn) {
 #pragma unroll
-  for (int i = 0; i < KC; i++)
-    if (i == idx) nT[i] += (int32_t)t;
+    for (int i = 0; i < KC; i++)
+      if (i == idx) nT[i] += (int32_t)t;
+  }
   if (kind == 0) q0 = q;
   if (kind == 1) {
     const double tt = __builtin_fma((double)(2 * q0), BETA_D, (double)(2 * q));
@@ -1578,26 +1615,11 @@ __device__ __forceinline__ void sqrt_lazy_step(const Group<L> &g, int64_t (&C)[K
   /* keep the scheduler from running later steps' digit chains ahead of this step's
    * multiply-adds (that holds several steps' columns live and costs occupancy) */
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (kRenorm) {
-    /* one carry level every 64 steps keeps the columns far below 2^63 */
-    if ((k & 63) == 63) {
-      int64_t car[KC];
-#pragma unroll
-      for (int i = 0; i < KC; i++) car[i] = C[i] >> DIV_RADIX;
-      const uint32_t clo = g.from_prev_row0((uint32_t)car[KC - 1]);
-      const uint32_t chi = g.from_prev_row0((uint32_t)((uint64_t)car[KC - 1] >> 32));
-      const int64_t cbelow = (int64_t)(((uint64_t)chi << 32) | clo);
-#pragma unroll
-      for (int i = KC - 1; i >= 0; i--) {
-        const int64_t keep = (top && i == KC - 1) ? C[i] : (C[i] & (int64_t)MASK);
-        C[i] = keep + (i > 0 ? car[i - 1] : cbelow);
-      }
-    }
-  }
+  return q;
 }
 
-template <int K, int L>
-__device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
+template <int K, int L, bool kShort>
+__device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea,
                                                const uint32_t (&A)[K], uint32_t &re, uint32_t (&Sr)[K]) {
   constexpr int n = K * L, p = 32 * n;
   constexpr int KD = div_digits_per_lane(K), F = KD * L, TX = DIV_RADIX * F - p;
@@ -1610,7 +1632,7 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     re = act;
 #pragma unroll
     for (int i = 0; i < K; i++) Sr[i] = 0;
-    return;
+    return true;
   }
   int64_t e = (int64_t)(ea >> 1) - FPA_BIAS_LL;
   const int o = (int)(e & 1);
@@ -1629,143 +1651,237 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   asm volatile("" : "+v"(nz0));
   double invs = 0.0;                       /* 2^27 / (T_{F-1} beta + T_{F-2}), from step 2 on */
   int32_t q0 = 0;
+  /* |q * 2q_j| < 2^57: one carry level at most every 64 steps keeps the columns below 2^63;
+   * it runs before a trip of n steps that would take the count past 64 */
+  int since = 0;
+  auto renorm_before = [&](auto &Cf, int n) __attribute__((always_inline)) {
+    if constexpr (kRen) {
+      if (since + n > 64) {
+        lazy_renorm(g, Cf, top);
+        since = 0;
+      }
+      since += n;
+    }
+  };
 
-  /* Phase 1, steps 0 .. HF-1: T fits the frame's top HF slots, and U's slots below them only
-   * carry radicand digits upwards, one per step.  These steps run on a half frame of the top
-   * HF = KH*L slots, KH columns per lane, the digits below entering at its bottom lane.
-   * LDS: [0, L) zeros, [L, L+F) the radicand's digits, [L+F, L+F+KH) zeros. */
-  uint32_t *dig = lds + L;
-  wsync();   /* the recoding reads of lds are done before the digits overwrite them */
-  lds[g.gl] = 0;
-#pragma unroll
-  for (int i = 0; i < KD; i++) dig[g.gl * KD + i] = ad[i];
-#pragma unroll
-  for (int q = 0; q < (KH + L - 1) / L; q++)
-    if (q * L + g.gl < KH) dig[F + q * L + g.gl] = 0;
-  wsync();
-  int64_t Ch[KH];
-  int32_t nTh[KH];
-#pragma unroll
-  for (int i = 0; i < KH; i++) { Ch[i] = (int64_t)dig[P2 + g.gl * KH + i]; nTh[i] = 0; }
-  {
-    /* digit P2-1-k enters the bottom lane at step k (zeros once k >= P2; they come from
-     * [0, L)); the other lanes read zeros */
-    uint32_t fl[KH];
-    auto fills = [&](int tr) __attribute__((always_inline)) {
-      const uint32_t *b = g.gl == 0 ? dig + (P2 - KH - tr * KH) : dig + F;
-#pragma unroll
-      for (int u = 0; u < KH; u++) fl[u] = b[KH - 1 - u];
-    };
-    /* trips of KH steps: step k's half-frame slot HF-1-k is index KH-1-u of lane L-1-trip;
-     * ownm: all ones in the lane that owns the trip's slots, opaque so that it is formed
-     * once per trip rather than again at every step */
-    fills(0);
-    uint32_t ownm = top ? ~0u : 0u;
-    asm volatile("" : "+v"(ownm));
-    repeat<KH>([&](int u) __attribute__((always_inline)) {
-      sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0,
-                                         fl[u], q0, invs);
-    });
-    for (int tr = 1; tr < L; tr++) {
-      fills(tr);
-      ownm = g.gl == L - 1 - tr ? ~0u : 0u;
-      asm volatile("" : "+v"(ownm));
-      repeat<KH>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KH, L, true, kRen>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
-      });
-    }
-  }
-  /* the half frame's slots P2 .. F-1 into the whole frame (KD columns per lane, zeros below
-   * P2), one 32-bit word per pass through LDS */
-  int64_t C[KD];
-  int32_t nT[KD];
-  {
-    wsync();
-#pragma unroll
-    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)Ch[i];
-#pragma unroll
-    for (int q = 0; q < (P2 + L - 1) / L; q++)
-      if (q * L + g.gl < P2) lds[q * L + g.gl] = 0;
-    wsync();
-    uint32_t lo[KD];
-#pragma unroll
-    for (int i = 0; i < KD; i++) lo[i] = lds[g.gl * KD + i];
-    wsync();
-#pragma unroll
-    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)((uint64_t)Ch[i] >> 32);
-    wsync();
-#pragma unroll
-    for (int i = 0; i < KD; i++) C[i] = (int64_t)(((uint64_t)lds[g.gl * KD + i] << 32) | lo[i]);
-    wsync();
-#pragma unroll
-    for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)nTh[i];
-    wsync();
-#pragma unroll
-    for (int i = 0; i < KD; i++) nT[i] = (int32_t)lds[g.gl * KD + i];
-  }
-  /* Phase 2, steps HF .. F-1 on the whole frame: slot F-1-k runs from P2-1 down to 0; the
-   * first trip covers the RT slots of lane LT, then whole trips of KD for lanes LT-1 .. 0 */
-  {
-    constexpr int LT = (P2 - 1) / KD, RT = P2 - LT * KD;
-    uint32_t ownm = g.gl == LT ? ~0u : 0u;
-    asm volatile("" : "+v"(ownm));
-    repeat<RT>([&](int u) __attribute__((always_inline)) {
-      sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
-    });
-    for (int tr = LT - 1; tr >= 0; tr--) {
-      ownm = g.gl == tr ? ~0u : 0u;
-      asm volatile("" : "+v"(ownm));
-      const int k0 = HF + RT + (LT - 1 - tr) * KD;
-      repeat<KD>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KD, L, false, kRen>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
-      });
-    }
-  }
-
-  /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
-  const double r = __builtin_fma(i64_to_f64(C[KD - 1]), BETA_D, i64_to_f64(C[KD - 2])) * invs * 0x1p-27;
-  const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
-  uint32_t rem_nz = 1;
-  int32_t delta = 0;
-  if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
-    int64_t Tn[KD];
-#pragma unroll
-    for (int i = 0; i < KD; i++) Tn[i] = -(int64_t)nT[i];
-    const int64_t Rtop = normalize_exact<KD, L>(g, C);
-    const int64_t Ttop = normalize_exact<KD, L>(g, Tn);
-    int cmp;                               /* sign of R - T */
-    if (Rtop != Ttop) {
-      cmp = Rtop > Ttop ? 1 : -1;
-    } else {
-      int c = 0;
-#pragma unroll
-      for (int i = KD - 1; i >= 0; i--)
-        if (c == 0 && C[i] != Tn[i]) c = C[i] > Tn[i] ? 1 : -1;
-      const uint64_t diff = g.ballot(c != 0);
-      cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
-    }
-    /* S += delta: R -> R - delta * T - 1 for delta = +-1 */
-    delta = Rtop < 0 ? -1 : cmp > 0 ? 1 : 0;
-    int64_t Rhi = Rtop;
-    if (delta != 0) {
-#pragma unroll
-      for (int i = 0; i < KD; i++) C[i] -= (int64_t)delta * Tn[i];
-      if (g.gl == 0) C[0] -= 1;
-      Rhi = normalize_exact<KD, L>(g, C);
-    }
-    /* R lies in [0, 2S], which reaches past beta^F: its part above the F digits counts */
-    uint32_t nz = 0;
-#pragma unroll
-    for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
-    rem_nz = (Rhi != 0 || g.any(nz != 0)) ? 1u : 0u;
-  }
-  /* root digits S_k = -nT/2 at slot k, then three zero words */
   int32_t *sd = reinterpret_cast<int32_t *>(lds);
-  wsync();
+  uint32_t rem_nz = 1;
+  if constexpr (kShort) {
+    /* Short frame: no step's digit and no end decision reads a slot below s = k - 1 after
+     * step k (the digits come from the top two columns, the end's estimate too, and slot s
+     * reaches slot F-2 only through F-2-s shifts), so those slots need not be kept.  The
+     * frame holds the top FB = KB*L >= (F+2)/2 slots, B0 = F - FB below it, for all F steps:
+     * until step F-1-B0 T fits in it and the slots below it only carry radicand digits
+     * upwards (they enter at its bottom lane, as in the half-frame phase of the full frame);
+     * after that the products and root digits below it land where nothing is read again,
+     * since 2*B0 <= F-2.  The digits are those of the full frame, and where the end's
+     * estimate is sure so is the result (S unchanged, a nonzero remainder); the root digits
+     * of the slots below the frame go to LDS as they are made.  Where some group of the
+     * wave is unsure, false: the caller runs the full frame.
+     * LDS: [0, ZB) zeros, [ZB, ZB+F) the radicand's digits, [ZB+F, ZB+F+KB) zeros; the
+     * root digits at [0, F) once the digits have entered. */
+    constexpr int KB = sqrt_short_cols(KD, L), FB = KB * L, B0 = F - FB, ZB = 2 * L + 2;
+    static_assert(sqrt_short_ok<K, L>(), "short frame shape");
+    uint32_t *dig = lds + ZB;
+    wsync();   /* the recoding reads of lds are done before the digits overwrite them */
+    for (int w = g.gl; w < ZB; w += L) lds[w] = 0;
 #pragma unroll
-  for (int i = 0; i < KD; i++) sd[g.gl * KD + i] = -nT[i] / 2 + (g.gl == 0 && i == 0 ? delta : 0);
-  if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
-  wsync();
+    for (int i = 0; i < KD; i++) dig[g.gl * KD + i] = ad[i];
+    for (int w = g.gl; w < KB; w += L) dig[F + w] = 0;
+    wsync();
+    int64_t Ch[KB];
+    int32_t nTh[KB];
+#pragma unroll
+    for (int i = 0; i < KB; i++) { Ch[i] = (int64_t)dig[B0 + g.gl * KB + i]; nTh[i] = 0; }
+    {
+      /* digit B0-1-k enters the bottom lane at step k (zeros once k >= B0; the lowest read,
+       * B0 - FB >= -2L-1, is in [0, ZB)); the other lanes read zeros */
+      uint32_t fl[KB];
+      auto fills = [&](int tr) __attribute__((always_inline)) {
+        const uint32_t *b = g.gl == 0 ? dig + (B0 - KB - tr * KB) : dig + F;
+#pragma unroll
+        for (int u = 0; u < KB; u++) fl[u] = b[KB - 1 - u];
+      };
+      fills(0);
+      uint32_t ownm = top ? ~0u : 0u;
+      asm volatile("" : "+v"(ownm));
+      renorm_before(Ch, KB);
+      repeat<KB>([&](int u) __attribute__((always_inline)) {
+        sqrt_lazy_step<KB, L, true>(g, Ch, nTh, u, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+                                    q0, invs);
+      });
+      for (int tr = 1; tr < L; tr++) {
+        fills(tr);
+        ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+        asm volatile("" : "+v"(ownm));
+        renorm_before(Ch, KB);
+        repeat<KB>([&](int u) __attribute__((always_inline)) {
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, tr * KB + u, KB - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+        });
+      }
+    }
+    /* steps FB .. F-1: slot F-1-k lies below the frame; its digit goes to LDS */
+    int k0 = FB;
+    for (; k0 + KB <= F; k0 += KB) {
+      renorm_before(Ch, KB);
+      repeat<KB>([&](int u) __attribute__((always_inline)) {
+        const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        sd[F - 1 - k0 - u] = q;
+      });
+    }
+    renorm_before(Ch, B0 % KB);
+    repeat<B0 % KB>([&](int u) __attribute__((always_inline)) {
+      const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+      sd[F - 1 - k0 - u] = q;
+    });
+    const double r = __builtin_fma(i64_to_f64(Ch[KB - 1]), BETA_D, i64_to_f64(Ch[KB - 2])) * invs * 0x1p-27;
+    const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
+    if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
+    wsync();
+#pragma unroll
+    for (int i = 0; i < KB; i++) sd[B0 + g.gl * KB + i] = -nTh[i] / 2;
+    if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
+    wsync();
+  } else {
+    /* Phase 1, steps 0 .. HF-1: T fits the frame's top HF slots, and U's slots below them only
+     * carry radicand digits upwards, one per step.  These steps run on a half frame of the top
+     * HF = KH*L slots, KH columns per lane, the digits below entering at its bottom lane.
+     * LDS: [0, L) zeros, [L, L+F) the radicand's digits, [L+F, L+F+KH) zeros. */
+    uint32_t *dig = lds + L;
+    wsync();   /* the recoding reads of lds are done before the digits overwrite them */
+    lds[g.gl] = 0;
+  #pragma unroll
+    for (int i = 0; i < KD; i++) dig[g.gl * KD + i] = ad[i];
+  #pragma unroll
+    for (int q = 0; q < (KH + L - 1) / L; q++)
+      if (q * L + g.gl < KH) dig[F + q * L + g.gl] = 0;
+    wsync();
+    int64_t Ch[KH];
+    int32_t nTh[KH];
+  #pragma unroll
+    for (int i = 0; i < KH; i++) { Ch[i] = (int64_t)dig[P2 + g.gl * KH + i]; nTh[i] = 0; }
+    {
+      /* digit P2-1-k enters the bottom lane at step k (zeros once k >= P2; they come from
+       * [0, L)); the other lanes read zeros */
+      uint32_t fl[KH];
+      auto fills = [&](int tr) __attribute__((always_inline)) {
+        const uint32_t *b = g.gl == 0 ? dig + (P2 - KH - tr * KH) : dig + F;
+  #pragma unroll
+        for (int u = 0; u < KH; u++) fl[u] = b[KH - 1 - u];
+      };
+      /* trips of KH steps: step k's half-frame slot HF-1-k is index KH-1-u of lane L-1-trip;
+       * ownm: all ones in the lane that owns the trip's slots, opaque so that it is formed
+       * once per trip rather than again at every step */
+      fills(0);
+      uint32_t ownm = top ? ~0u : 0u;
+      asm volatile("" : "+v"(ownm));
+      renorm_before(Ch, KH);
+      repeat<KH>([&](int u) __attribute__((always_inline)) {
+        sqrt_lazy_step<KH, L, true>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+                                    q0, invs);
+      });
+      for (int tr = 1; tr < L; tr++) {
+        fills(tr);
+        ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+        asm volatile("" : "+v"(ownm));
+        renorm_before(Ch, KH);
+        repeat<KH>([&](int u) __attribute__((always_inline)) {
+          sqrt_lazy_step<KH, L, true>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+        });
+      }
+    }
+    /* the half frame's slots P2 .. F-1 into the whole frame (KD columns per lane, zeros below
+     * P2), one 32-bit word per pass through LDS */
+    int64_t C[KD];
+    int32_t nT[KD];
+    {
+      wsync();
+  #pragma unroll
+      for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)Ch[i];
+  #pragma unroll
+      for (int q = 0; q < (P2 + L - 1) / L; q++)
+        if (q * L + g.gl < P2) lds[q * L + g.gl] = 0;
+      wsync();
+      uint32_t lo[KD];
+  #pragma unroll
+      for (int i = 0; i < KD; i++) lo[i] = lds[g.gl * KD + i];
+      wsync();
+  #pragma unroll
+      for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)((uint64_t)Ch[i] >> 32);
+      wsync();
+  #pragma unroll
+      for (int i = 0; i < KD; i++) C[i] = (int64_t)(((uint64_t)lds[g.gl * KD + i] << 32) | lo[i]);
+      wsync();
+  #pragma unroll
+      for (int i = 0; i < KH; i++) lds[P2 + g.gl * KH + i] = (uint32_t)nTh[i];
+      wsync();
+  #pragma unroll
+      for (int i = 0; i < KD; i++) nT[i] = (int32_t)lds[g.gl * KD + i];
+    }
+    /* Phase 2, steps HF .. F-1 on the whole frame: slot F-1-k runs from P2-1 down to 0; the
+     * first trip covers the RT slots of lane LT, then whole trips of KD for lanes LT-1 .. 0 */
+    {
+      constexpr int LT = (P2 - 1) / KD, RT = P2 - LT * KD;
+      uint32_t ownm = g.gl == LT ? ~0u : 0u;
+      asm volatile("" : "+v"(ownm));
+      renorm_before(C, RT);
+      repeat<RT>([&](int u) __attribute__((always_inline)) {
+        sqrt_lazy_step<KD, L, false>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+      });
+      for (int tr = LT - 1; tr >= 0; tr--) {
+        ownm = g.gl == tr ? ~0u : 0u;
+        asm volatile("" : "+v"(ownm));
+        const int k0 = HF + RT + (LT - 1 - tr) * KD;
+        renorm_before(C, KD);
+        repeat<KD>([&](int u) __attribute__((always_inline)) {
+          sqrt_lazy_step<KD, L, false>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+        });
+      }
+    }
+
+    /* R = sum C_k beta^k should lie in [0, T]; its estimate r ~ R / T */
+    const double r = __builtin_fma(i64_to_f64(C[KD - 1]), BETA_D, i64_to_f64(C[KD - 2])) * invs * 0x1p-27;
+    const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
+    int32_t delta = 0;
+    if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
+      int64_t Tn[KD];
+  #pragma unroll
+      for (int i = 0; i < KD; i++) Tn[i] = -(int64_t)nT[i];
+      const int64_t Rtop = normalize_exact<KD, L>(g, C);
+      const int64_t Ttop = normalize_exact<KD, L>(g, Tn);
+      int cmp;                               /* sign of R - T */
+      if (Rtop != Ttop) {
+        cmp = Rtop > Ttop ? 1 : -1;
+      } else {
+        int c = 0;
+  #pragma unroll
+        for (int i = KD - 1; i >= 0; i--)
+          if (c == 0 && C[i] != Tn[i]) c = C[i] > Tn[i] ? 1 : -1;
+        const uint64_t diff = g.ballot(c != 0);
+        cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
+      }
+      /* S += delta: R -> R - delta * T - 1 for delta = +-1 */
+      delta = Rtop < 0 ? -1 : cmp > 0 ? 1 : 0;
+      int64_t Rhi = Rtop;
+      if (delta != 0) {
+  #pragma unroll
+        for (int i = 0; i < KD; i++) C[i] -= (int64_t)delta * Tn[i];
+        if (g.gl == 0) C[0] -= 1;
+        Rhi = normalize_exact<KD, L>(g, C);
+      }
+      /* R lies in [0, 2S], which reaches past beta^F: its part above the F digits counts */
+      uint32_t nz = 0;
+  #pragma unroll
+      for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
+      rem_nz = (Rhi != 0 || g.any(nz != 0)) ? 1u : 0u;
+    }
+    /* root digits S_k = -nT/2 at slot k, then three zero words */
+    wsync();
+  #pragma unroll
+    for (int i = 0; i < KD; i++) sd[g.gl * KD + i] = -nT[i] / 2 + (g.gl == 0 && i == 0 ? delta : 0);
+    if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
+    wsync();
+  }
   {
     bool bad = false;
 #pragma unroll
@@ -1803,6 +1919,7 @@ __device__ __forceinline__ void core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   if (round_mantissa<K, L>(g, Mw, mode, 0u, rw, sticky)) E += 1;
   re = pack_exp(E, 0u, 0, 0x7FFFFFFFll);
   chunk_select(Sr, true, Mw, Mw);
+  return true;
 }
 
 }  // namespace fpa
