@@ -1149,10 +1149,38 @@ __host__ __device__ constexpr int div_pad_words(int K, int L) {
   return (DIV_RADIX * div_digits_per_lane(K) * L - 32 * K * L + 31) / 32;
 }
 
-/* LDS words per value: recoding stage or the quotient digits (+ topbit, + two zeros) */
+/* The short division's phases: TA trips of KD steps on the whole frame (JA steps), then S-JA
+ * steps on the top FB = KB*L slots (B below them).  After step j no later digit and not the
+ * end's estimate reads a slot below j - C1 (slot s reaches slot ND-2, where the estimates
+ * read, only through ND-2-s shifts), so the frame of the second phase is the slots from
+ * JA-2-C1 up (one slot of margin).  Its relayout passes through LDS words [0, FB), below the
+ * quotient digits, which move up by X words to leave them free. */
+struct DivShortShape {
+  int KD, ND, S, TA, JA, C1, KB, FB, B, X;
+  bool ok;
+};
+__host__ __device__ constexpr DivShortShape div_short_shape(int K, int L) {
+  DivShortShape d{};
+  d.KD = div_digits_per_lane(K);
+  d.ND = d.KD * L;
+  d.S = div_steps(K * L);
+  d.TA = (d.S + d.KD) / (2 * d.KD);
+  d.JA = d.TA * d.KD;
+  d.C1 = d.S - d.ND + 1;
+  d.KB = (d.ND - d.JA + 2 + d.C1 + L - 1) / L;
+  d.FB = d.KB * L;
+  d.B = d.ND - d.FB;
+  d.X = d.FB > d.S - d.JA ? d.FB - (d.S - d.JA) : 0;
+  d.ok = d.TA >= 1 && d.JA < d.S && d.KB >= 2 && d.KB < d.KD && d.B >= 0 && d.B <= d.JA - 2 - d.C1;
+  return d;
+}
+
+/* LDS words per value: recoding stage or the quotient digits (+ topbit, + two zeros; X words
+ * up in the short division) */
 __host__ __device__ constexpr int div_lds_words(int K, int L) {
-  return div_pad_words(K, L) + K * L + 2 > div_steps(K * L) + 3 ? div_pad_words(K, L) + K * L + 2
-                                                                   : div_steps(K * L) + 3;
+  return div_pad_words(K, L) + K * L + 2 > div_short_shape(K, L).X + div_steps(K * L) + 3
+             ? div_pad_words(K, L) + K * L + 2
+             : div_short_shape(K, L).X + div_steps(K * L) + 3;
 }
 
 /* Columns per lane of the short sqrt's frame: the top FB = KB*L >= (F + 2) / 2 slots of the
@@ -1351,9 +1379,27 @@ __device__ __forceinline__ int64_t normalize_exact(const Group<L> &g, int64_t (&
   return bcast64(g, carry, L - 1);
 }
 
-/* (re, Qr) = a / b; lds: div_lds_words(K, L) words */
-template <int K, int L>
-__device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
+/* One digit step of the lazy division on a frame of KC columns per lane: R <- R * beta, the
+ * digit from the top two columns (top lane) broadcast, R -= q * D, q to *qd */
+template <int KC, int L>
+__device__ __forceinline__ void div_lazy_step(const Group<L> &g, int64_t (&C)[KC], const int32_t (&nd)[KC], int32_t *qd,
+                                              uint32_t nz0, bool top, double invs) {
+  lazy_shift_up<KC, L>(g, C, nz0, top);
+  const int32_t q = (int32_t)g.bcast_top((uint32_t)lazy_top_digit(C[KC - 1], C[KC - 2], invs));
+#pragma unroll
+  for (int i = 0; i < KC; i++) C[i] = (int64_t)q * nd[i] + C[i];
+  *qd = q;
+  /* keep the scheduler from running the digit chain of later steps ahead of this step's
+   * multiply-adds: that lookahead holds several steps' columns live at once and costs
+   * occupancy, while other waves hide the chain's latency anyway */
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+/* (re, Qr) = a / b; lds: div_lds_words(K, L) words.  kShort: the short division (second
+ * phase on the top slots only); false where the end's estimate leaves some group of the wave
+ * unsure, and the caller then runs the whole frame. */
+template <int K, int L, bool kShort>
+__device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, int mode, uint32_t ea, const uint32_t (&A)[K],
                               uint32_t eb, const uint32_t (&B)[K], uint32_t &re, uint32_t (&Qr)[K]) {
   constexpr int n = K * L, p = 32 * n;
   constexpr int KD = div_digits_per_lane(K), ND = KD * L, S = div_steps(n);
@@ -1366,7 +1412,7 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
     re = act;
 #pragma unroll
     for (int i = 0; i < K; i++) Qr[i] = 0;
-    return;
+    return true;
   }
   const uint32_t sign = (ea ^ eb) & 1u;
   int64_t E = (int64_t)(ea >> 1) - (int64_t)(eb >> 1) + FPA_BIAS_LL;
@@ -1403,64 +1449,112 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
   const bool top = g.gl == L - 1;
   uint32_t nz0 = g.gl == 0 ? 0u : ~0u;
   asm volatile("" : "+v"(nz0));
-  int32_t *qd = reinterpret_cast<int32_t *>(lds);   /* quotient digit of step j at S-1-j */
-
-  auto step = [&](int j) __attribute__((always_inline)) {
-    /* R <- R * beta */
-    lazy_shift_up<KD, L>(g, C, nz0, top);
-    /* digit estimate from the top two columns (top lane), broadcast */
-    const int32_t q = (int32_t)g.bcast_top((uint32_t)lazy_top_digit(C[KD - 1], C[KD - 2], invs));
-#pragma unroll
-    for (int i = 0; i < KD; i++) C[i] = (int64_t)q * nd[i] + C[i];
-    qd[S - 1 - j] = q;
-    /* keep the scheduler from running the digit chain of later steps ahead of this step's
-     * multiply-adds: that lookahead holds several steps' columns live at once and costs
-     * occupancy, while other waves hide the chain's latency anyway */
-    __builtin_amdgcn_sched_barrier(0);
-  };
+  constexpr DivShortShape SH = div_short_shape(K, L);
+  static_assert(!kShort || SH.ok, "short division shape");
+  /* quotient digit of step j at S-1-j (X words up in the short division) */
+  int32_t *qd = reinterpret_cast<int32_t *>(lds) + (kShort ? SH.X : 0);
+  /* one carry level at most every 128 steps keeps the columns far below 2^63 (frames of
+   * more than 255 slots; a frame of at most 255 slots takes at most 255 products of < 2^55
+   * per column) */
+  constexpr int RT = 128 / KD;
   wsync();   /* the recoding reads of lds are done before the digits overwrite them */
   int j0 = 0;
-  for (; j0 + KD <= S; j0 += KD) {
+  for (; j0 + KD <= (kShort ? SH.JA : S); j0 += KD) {
     if constexpr (ND > 255) {
-      /* one carry level at most every 128 steps keeps the columns far below 2^63 */
-      constexpr int RT = 128 / KD;
       if (j0 > 0 && (j0 / KD) % RT == 0) lazy_renorm<KD, L>(g, C, top);
     }
-    repeat<KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
+    repeat<KD>([&](int u) __attribute__((always_inline)) {
+      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, top, invs);
+    });
   }
-  repeat<S % KD>([&](int u) __attribute__((always_inline)) { step(j0 + u); });
-
-  /* final remainder R = sum C_k beta^k lies in (-D, 2D); its estimate t ~ R/D */
-  const double t = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2])) * (invs * 0x1p-27);
-  const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
+  if constexpr (kShort) {
+    /* the slots from B up into a frame of KB columns per lane (lane gl: B + gl*KB ..), one
+     * 32-bit word per pass through LDS [0, FB); lane LB holds slot B at index IB */
+    constexpr int KB = SH.KB, FB = SH.FB, SB = SH.B, LB = SB / KD, IB = SB - LB * KD;
+    int64_t Cb[KB];
+    int32_t ndb[KB];
+    uint32_t *rl = lds;
+    auto relayout = [&](auto get, auto set) __attribute__((always_inline)) {
+      wsync();
+      if (g.gl > LB) {
+#pragma unroll
+        for (int i = 0; i < KD; i++) rl[g.gl * KD - SB + i] = get(i);
+      } else if (g.gl == LB) {
+#pragma unroll
+        for (int i = IB; i < KD; i++) rl[i - IB] = get(i);
+      }
+      wsync();
+#pragma unroll
+      for (int i = 0; i < KB; i++) set(i, rl[g.gl * KB + i]);
+    };
+    relayout([&](int i) { return (uint32_t)C[i]; }, [&](int i, uint32_t w) { Cb[i] = (int64_t)w; });
+    relayout([&](int i) { return (uint32_t)((uint64_t)C[i] >> 32); },
+             [&](int i, uint32_t w) { Cb[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Cb[i]); });
+    relayout([&](int i) { return (uint32_t)nd[i]; }, [&](int i, uint32_t w) { ndb[i] = (int32_t)w; });
+    static_assert(FB <= SH.X + S - SH.JA, "the relayout stays below the quotient digits");
+    int since = 0;
+    auto renorm_before = [&](int n) __attribute__((always_inline)) {
+      if constexpr (ND > 255) {
+        if (since + n > 128) {
+          lazy_renorm<KB, L>(g, Cb, top);
+          since = 0;
+        }
+        since += n;
+      }
+    };
+    for (; j0 + KB <= S; j0 += KB) {
+      renorm_before(KB);
+      repeat<KB>([&](int u) __attribute__((always_inline)) {
+        div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
+      });
+    }
+    renorm_before((S - SH.JA) % KB);
+    repeat<(S - SH.JA) % KB>([&](int u) __attribute__((always_inline)) {
+      div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
+    });
+    /* the end's estimate as in the whole frame; sure: that frame would keep the digits and
+     * find a nonzero remainder */
+    const double t = __builtin_fma(i64_to_f64(Cb[KB - 1]), (double)BETA, i64_to_f64(Cb[KB - 2])) * (invs * 0x1p-27);
+    const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
+    if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
+  } else {
+    repeat<S % KD>([&](int u) __attribute__((always_inline)) {
+      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, top, invs);
+    });
+  }
   uint32_t rem_nz = 1;
   int32_t delta = 0;
-  if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
-    /* exact: normalise R; add or subtract D once; the quotient moves by -1 / +1 */
-    const int64_t T = normalize_exact<KD, L>(g, C);
-    int64_t sgn = 0;                       /* +1: subtract D, -1: add D */
-    if (T < 0) {
-      sgn = -1;
-    } else if (T > 0) {
-      sgn = 1;
-    } else {
-      /* compare R with D: the highest lane with a differing digit decides */
-      int c = 0;
+  if constexpr (!kShort) {
+    /* final remainder R = sum C_k beta^k lies in (-D, 2D); its estimate t ~ R/D */
+    const double t = __builtin_fma(i64_to_f64(C[KD - 1]), (double)BETA, i64_to_f64(C[KD - 2])) * (invs * 0x1p-27);
+    const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
+    if (__builtin_expect(__ballot(top && unsure) != 0, 0)) {
+      /* exact: normalise R; add or subtract D once; the quotient moves by -1 / +1 */
+      const int64_t T = normalize_exact<KD, L>(g, C);
+      int64_t sgn = 0;                       /* +1: subtract D, -1: add D */
+      if (T < 0) {
+        sgn = -1;
+      } else if (T > 0) {
+        sgn = 1;
+      } else {
+        /* compare R with D: the highest lane with a differing digit decides */
+        int c = 0;
 #pragma unroll
-      for (int i = KD - 1; i >= 0; i--)
-        if (c == 0 && C[i] != (int64_t)-nd[i]) c = C[i] > (int64_t)-nd[i] ? 1 : -1;
-      const uint64_t diff = g.ballot(c != 0);
-      const int cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
-      sgn = cmp >= 0 ? 1 : 0;
+        for (int i = KD - 1; i >= 0; i--)
+          if (c == 0 && C[i] != (int64_t)-nd[i]) c = C[i] > (int64_t)-nd[i] ? 1 : -1;
+        const uint64_t diff = g.ballot(c != 0);
+        const int cmp = diff ? (int)g.bcast((uint32_t)c, 63 - __clzll(diff)) : 0;
+        sgn = cmp >= 0 ? 1 : 0;
+      }
+      delta = (int32_t)sgn;
+#pragma unroll
+      for (int i = 0; i < KD; i++) C[i] += sgn * (int64_t)nd[i];
+      normalize_exact<KD, L>(g, C);
+      uint32_t nz = 0;
+#pragma unroll
+      for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
+      rem_nz = g.any(nz != 0) ? 1u : 0u;
     }
-    delta = (int32_t)sgn;
-#pragma unroll
-    for (int i = 0; i < KD; i++) C[i] += sgn * (int64_t)nd[i];
-    normalize_exact<KD, L>(g, C);
-    uint32_t nz = 0;
-#pragma unroll
-    for (int i = 0; i < KD; i++) nz |= (uint32_t)C[i];
-    rem_nz = g.any(nz != 0) ? 1u : 0u;
   }
   wsync();
   if (delta != 0 && g.gl == 0) qd[0] += delta;
@@ -1502,6 +1596,7 @@ __device__ __forceinline__ void core_div_lazy(const Group<L> &g, uint32_t *lds, 
   if (round_mantissa<K, L>(g, Mw, mode, sign, rw, sticky)) E += 1;
   re = pack_exp(E, sign, 10, 0x7FFFFFFEll);
   chunk_select(Qr, true, Mw, Mw);
+  return true;
 }
 
 template <int K, int L>
@@ -1509,7 +1604,19 @@ __device__ __forceinline__ void op_div(const fpa_arguments &a, const Slot &s, co
   uint32_t A[K], B[K], R[K], re;
   load_value<K, L>(g, a.a_limbs, s, A);
   load_value<K, L>(g, a.b_limbs, s, B);
-  core_div_lazy<K, L>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  if constexpr (div_short_shape(K, L).ok) {
+    if (!core_div_lazy<K, L, true>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R)) {
+      /* some group of the wave was left unsure: the whole wave runs the whole frame, from
+       * the operands reloaded (the clobber keeps the compiler from holding the first loads'
+       * registers through the short run instead) */
+      asm volatile("" ::: "memory");
+      load_value<K, L>(g, a.a_limbs, s, A);
+      load_value<K, L>(g, a.b_limbs, s, B);
+      core_div_lazy<K, L, false>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+    }
+  } else {
+    core_div_lazy<K, L, false>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R);
+  }
   store_result(a, s, g.gl, re, R);
 }
 
