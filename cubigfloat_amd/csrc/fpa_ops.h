@@ -13,10 +13,12 @@
  *            normalised once at the end.
  *   div      fpa_warp_div.h:127-201 -> op_div: long division in radix 2^27 with the
  *            remainder kept as lazy signed 64-bit column sums (one multiply-add per column
- *            and digit), FP64 digit estimates, redundant digits, exact end correction.
+ *            and digit), FP64 digit estimates, redundant digits; the second half of the
+ *            steps on the top slots only (short division), with the whole frame and its
+ *            exact end correction as the fallback where the end's estimate is unsure.
  *   sqrt     fpa_warp_sqrt.h:195-254 -> op_sqrt: digit-by-digit square root in radix
- *            2^27 on div's lazy column frame (core_sqrt_lazy), the first half of the
- *            digits on a half frame.
+ *            2^27 on div's lazy column frame (core_sqrt_lazy), all steps on the top half
+ *            of the frame (short root), the half-then-whole frame as the fallback.
  * Rounding uses fpa_round's decision (fpa_scalar.h) on the exact (round word, sticky)
  * pair; a carry out of the rounding increment becomes 0x80..0 one binade up.
  */
