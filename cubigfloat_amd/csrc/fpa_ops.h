@@ -25,6 +25,8 @@
 #ifndef FPA_OPS_H
 #define FPA_OPS_H
 
+#include <type_traits>
+
 #include "fpa_group.h"
 #include "fpa_mul_sched.h"
 #include "../../include/fpa_kernels.h"
@@ -1274,6 +1276,33 @@ __device__ __forceinline__ void lazy_renorm(const Group<L> &g, int64_t (&C)[KC],
   }
 }
 
+/* One 32-bit word per slot moved from one lazy frame into another through LDS: the slots of
+ * a frame of KC1 columns per lane whose bottom slot is B1 (lane gl holds B1 + gl*KC1 ..) go to
+ * R[slot - SB] where slot >= SB; where the new frame reaches below B1 (SB < B1), 'pre' writes
+ * R[0, B1 - SB); then lane gl takes the new frame's column i (slot SB + gl*KC2 + i) from
+ * R[gl*KC2 + i].  get(i) gives the old column i's word, set(i, w) takes the new one's. */
+template <int KC1, int KC2, int B1, int SB, int L, class Get, class Pre, class Set>
+__device__ __forceinline__ void frame_relayout(const Group<L> &g, uint32_t *R, Get get, Pre pre, Set set) {
+  wsync();
+  if constexpr (SB <= B1) {
+#pragma unroll
+    for (int i = 0; i < KC1; i++) R[B1 - SB + g.gl * KC1 + i] = get(i);
+    pre();
+  } else {
+    constexpr int LB = (SB - B1) / KC1, IB = (SB - B1) - LB * KC1;
+    if (g.gl > LB) {
+#pragma unroll
+      for (int i = 0; i < KC1; i++) R[B1 - SB + g.gl * KC1 + i] = get(i);
+    } else if (g.gl == LB) {
+#pragma unroll
+      for (int i = IB; i < KC1; i++) R[i - IB] = get(i);
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int i = 0; i < KC2; i++) set(i, R[g.gl * KC2 + i]);
+}
+
 /* Digits of X (n = K*L words) in radix 2^27 aligned to the top of KD*L digits: digit k of
  * the value = bits [27k - PAD + extra, +27) of X, PAD = 27*KD*L - 32n (bits below X's bit
  * 0 read as zero).  lds: ZW + n + 2 words, ZW = ceil(PAD / 32). */
@@ -1472,27 +1501,16 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
   if constexpr (kShort) {
     /* the slots from B up into a frame of KB columns per lane (lane gl: B + gl*KB ..), one
      * 32-bit word per pass through LDS [0, FB); lane LB holds slot B at index IB */
-    constexpr int KB = SH.KB, FB = SH.FB, SB = SH.B, LB = SB / KD, IB = SB - LB * KD;
+    constexpr int KB = SH.KB, FB = SH.FB, SB = SH.B;
     int64_t Cb[KB];
     int32_t ndb[KB];
-    uint32_t *rl = lds;
-    auto relayout = [&](auto get, auto set) __attribute__((always_inline)) {
-      wsync();
-      if (g.gl > LB) {
-#pragma unroll
-        for (int i = 0; i < KD; i++) rl[g.gl * KD - SB + i] = get(i);
-      } else if (g.gl == LB) {
-#pragma unroll
-        for (int i = IB; i < KD; i++) rl[i - IB] = get(i);
-      }
-      wsync();
-#pragma unroll
-      for (int i = 0; i < KB; i++) set(i, rl[g.gl * KB + i]);
-    };
-    relayout([&](int i) { return (uint32_t)C[i]; }, [&](int i, uint32_t w) { Cb[i] = (int64_t)w; });
-    relayout([&](int i) { return (uint32_t)((uint64_t)C[i] >> 32); },
-             [&](int i, uint32_t w) { Cb[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Cb[i]); });
-    relayout([&](int i) { return (uint32_t)nd[i]; }, [&](int i, uint32_t w) { ndb[i] = (int32_t)w; });
+    auto none = [] {};
+    frame_relayout<KD, KB, 0, SB, L>(g, lds, [&](int i) { return (uint32_t)C[i]; }, none,
+                                     [&](int i, uint32_t w) { Cb[i] = (int64_t)w; });
+    frame_relayout<KD, KB, 0, SB, L>(g, lds, [&](int i) { return (uint32_t)((uint64_t)C[i] >> 32); }, none,
+                                     [&](int i, uint32_t w) { Cb[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Cb[i]); });
+    frame_relayout<KD, KB, 0, SB, L>(g, lds, [&](int i) { return (uint32_t)nd[i]; }, none,
+                                     [&](int i, uint32_t w) { ndb[i] = (int32_t)w; });
     static_assert(FB <= SH.X + S - SH.JA, "the relayout stays below the quotient digits");
     int since = 0;
     auto renorm_before = [&](int n) __attribute__((always_inline)) {
@@ -1791,6 +1809,20 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
      * root digits at [0, F) once the digits have entered. */
     constexpr int KB = sqrt_short_cols(KD, L), FB = KB * L, B0 = F - FB, ZB = 2 * L + 2;
     static_assert(sqrt_short_ok<K, L>(), "short frame shape");
+    /* Where KB and L are even and the frame is long (F >= 300, or F >= 150 with lanes of at
+     * most 24 limbs: for shorter frames, and for 32-limb lanes of 152 slots, the two more
+     * relayouts cost more than they save), the first and last steps run on frames of KB/2
+     * columns per lane (the top FB/2 slots): steps 0 .. FB/2-1 on the top FB/2 slots (T fits them, the
+     * slots below only carry radicand digits), and from step K2 = B3 + 3 on the top FB/2 slots
+     * again (B3 = F - FB/2: no later digit reads a slot below k - 1).  Between the frames the
+     * slots pass through LDS: first into dig[B0, F), whose radicand digits have all entered the
+     * frame by then, with the digits still to enter below the first frame taken from
+     * dig[B0 - FB/2, B0); then into the root digits' LDS words [B3, F), which are written
+     * last. */
+    constexpr bool kQ = KB % 2 == 0 && L % 2 == 0 && KB / 2 >= 2 && (F >= 300 || (F >= 150 && K <= 24));
+    constexpr int KQ = kQ ? KB / 2 : KB, FQ = KQ * L, B1 = F - FQ, B3 = F - FQ, K1 = kQ ? FQ : 0;
+    constexpr int K2 = kQ ? B3 + 3 : F;
+    static_assert(!kQ || (K1 % KB == 0 && K2 >= FB && 2 * FQ < F + 4), "quarter frames");
     uint32_t *dig = lds + ZB;
     wsync();   /* the recoding reads of lds are done before the digits overwrite them */
     for (int w = g.gl; w < ZB; w += L) lds[w] = 0;
@@ -1800,28 +1832,81 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     wsync();
     int64_t Ch[KB];
     int32_t nTh[KB];
-#pragma unroll
-    for (int i = 0; i < KB; i++) { Ch[i] = (int64_t)dig[B0 + g.gl * KB + i]; nTh[i] = 0; }
     {
-      /* digit B0-1-k enters the bottom lane at step k (zeros once k >= B0; the lowest read,
-       * B0 - FB >= -2L-1, is in [0, ZB)); the other lanes read zeros */
+      /* digit B-1-k enters the bottom lane of a frame with bottom slot B at step k (zeros once
+       * k >= B; the lowest read, B0 - FB >= -2L-1, is in [0, ZB)); the other lanes read zeros */
       uint32_t fl[KB];
-      auto fills = [&](int tr) __attribute__((always_inline)) {
-        const uint32_t *b = g.gl == 0 ? dig + (B0 - KB - tr * KB) : dig + F;
+      auto fills = [&](int tr, auto kc, int b) __attribute__((always_inline)) {
+        constexpr int KC = decltype(kc)::value;
+        const uint32_t *src = g.gl == 0 ? dig + (b - KC - tr * KC) : dig + F;
 #pragma unroll
-        for (int u = 0; u < KB; u++) fl[u] = b[KB - 1 - u];
+        for (int u = 0; u < KC; u++) fl[u] = src[KC - 1 - u];
       };
-      fills(0);
-      uint32_t ownm = top ? ~0u : 0u;
-      asm volatile("" : "+v"(ownm));
-      renorm_before(Ch, KB);
-      repeat<KB>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KB, L, true>(g, Ch, nTh, u, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
-                                    q0, invs);
-      });
-      for (int tr = 1; tr < L; tr++) {
-        fills(tr);
-        ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+      int tr0 = 0;
+      if constexpr (kQ) {
+        int64_t Cq[KQ];
+        int32_t nTq[KQ];
+#pragma unroll
+        for (int i = 0; i < KQ; i++) { Cq[i] = (int64_t)dig[B1 + g.gl * KQ + i]; nTq[i] = 0; }
+        using KQc = std::integral_constant<int, KQ>;
+        fills(0, KQc{}, B1);
+        uint32_t ownm = top ? ~0u : 0u;
+        asm volatile("" : "+v"(ownm));
+        renorm_before(Cq, KQ);
+        repeat<KQ>([&](int u) __attribute__((always_inline)) {
+          sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, u, KQ - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+                                      q0, invs);
+        });
+        for (int tr = 1; tr < L; tr++) {
+          fills(tr, KQc{}, B1);
+          ownm = g.gl == L - 1 - tr ? ~0u : 0u;
+          asm volatile("" : "+v"(ownm));
+          renorm_before(Cq, KQ);
+          repeat<KQ>([&](int u) __attribute__((always_inline)) {
+            sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, tr * KQ + u, KQ - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+          });
+        }
+        /* into the frame of the top FB slots: below B1 the radicand digits not yet entered
+         * (slot s holds the digit first at slot s - K1) */
+        uint32_t *R = dig + B0;
+        frame_relayout<KQ, KB, B1, B0, L>(
+            g, R, [&](int i) { return (uint32_t)Cq[i]; },
+            [&] {
+              for (int w = g.gl; w < B1 - B0; w += L) R[w] = dig[B0 - K1 + w];
+            },
+            [&](int i, uint32_t w) { Ch[i] = (int64_t)w; });
+        frame_relayout<KQ, KB, B1, B0, L>(
+            g, R, [&](int i) { return (uint32_t)((uint64_t)Cq[i] >> 32); },
+            [&] {
+              for (int w = g.gl; w < B1 - B0; w += L) R[w] = 0;
+            },
+            [&](int i, uint32_t w) { Ch[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Ch[i]); });
+        frame_relayout<KQ, KB, B1, B0, L>(
+            g, R, [&](int i) { return (uint32_t)nTq[i]; },
+            [&] {
+              for (int w = g.gl; w < B1 - B0; w += L) R[w] = 0;
+            },
+            [&](int i, uint32_t w) { nTh[i] = (int32_t)w; });
+        tr0 = K1 / KB;
+      } else {
+#pragma unroll
+        for (int i = 0; i < KB; i++) { Ch[i] = (int64_t)dig[B0 + g.gl * KB + i]; nTh[i] = 0; }
+      }
+      using KBc = std::integral_constant<int, KB>;
+      if (tr0 == 0) {
+        fills(0, KBc{}, B0);
+        uint32_t ownm = top ? ~0u : 0u;
+        asm volatile("" : "+v"(ownm));
+        renorm_before(Ch, KB);
+        repeat<KB>([&](int u) __attribute__((always_inline)) {
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, u, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+                                      q0, invs);
+        });
+        tr0 = 1;
+      }
+      for (int tr = tr0; tr < L; tr++) {
+        fills(tr, KBc{}, B0);
+        uint32_t ownm = g.gl == L - 1 - tr ? ~0u : 0u;
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KB);
         repeat<KB>([&](int u) __attribute__((always_inline)) {
@@ -1829,28 +1914,75 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         });
       }
     }
-    /* steps FB .. F-1: slot F-1-k lies below the frame; its digit goes to LDS */
+    /* steps FB .. K2-1: slot F-1-k lies below the frame; its digit goes to LDS */
     int k0 = FB;
-    for (; k0 + KB <= F; k0 += KB) {
+    for (; k0 + KB <= K2; k0 += KB) {
       renorm_before(Ch, KB);
       repeat<KB>([&](int u) __attribute__((always_inline)) {
         const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
         sd[F - 1 - k0 - u] = q;
       });
     }
-    renorm_before(Ch, B0 % KB);
-    repeat<B0 % KB>([&](int u) __attribute__((always_inline)) {
+    renorm_before(Ch, (K2 - FB) % KB);
+    repeat<(K2 - FB) % KB>([&](int u) __attribute__((always_inline)) {
       const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
       sd[F - 1 - k0 - u] = q;
     });
-    const double r = __builtin_fma(i64_to_f64(Ch[KB - 1]), BETA_D, i64_to_f64(Ch[KB - 2])) * invs * 0x1p-27;
-    const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
-    if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
-    wsync();
+    if constexpr (kQ) {
+      /* steps K2 .. F-1 on the top FQ slots: the root digits of slots B0 .. B3-1 go to LDS
+       * first, then the frame moves through the root digits' words [B3, F) */
+      int64_t Cq[KQ];
+      int32_t nTq[KQ];
+      {
+        wsync();
+        constexpr int LQ = (B3 - B0) / KB, IQ = (B3 - B0) - LQ * KB;
+        if (g.gl < LQ) {
 #pragma unroll
-    for (int i = 0; i < KB; i++) sd[B0 + g.gl * KB + i] = -nTh[i] / 2;
-    if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
-    wsync();
+          for (int i = 0; i < KB; i++) sd[B0 + g.gl * KB + i] = -nTh[i] / 2;
+        } else if (g.gl == LQ) {
+#pragma unroll
+          for (int i = 0; i < IQ; i++) sd[B0 + g.gl * KB + i] = -nTh[i] / 2;
+        }
+        uint32_t *R = reinterpret_cast<uint32_t *>(sd) + B3;
+        auto none = [] {};
+        frame_relayout<KB, KQ, B0, B3, L>(g, R, [&](int i) { return (uint32_t)Ch[i]; }, none,
+                                          [&](int i, uint32_t w) { Cq[i] = (int64_t)w; });
+        frame_relayout<KB, KQ, B0, B3, L>(g, R, [&](int i) { return (uint32_t)((uint64_t)Ch[i] >> 32); }, none,
+                                          [&](int i, uint32_t w) { Cq[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Cq[i]); });
+        frame_relayout<KB, KQ, B0, B3, L>(g, R, [&](int i) { return (uint32_t)nTh[i]; }, none,
+                                          [&](int i, uint32_t w) { nTq[i] = (int32_t)w; });
+      }
+      int k3 = K2;
+      for (; k3 + KQ <= F; k3 += KQ) {
+        renorm_before(Cq, KQ);
+        repeat<KQ>([&](int u) __attribute__((always_inline)) {
+          const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, k3 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+          sd[F - 1 - k3 - u] = q;
+        });
+      }
+      renorm_before(Cq, (F - K2) % KQ);
+      repeat<(F - K2) % KQ>([&](int u) __attribute__((always_inline)) {
+        const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, k3 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        sd[F - 1 - k3 - u] = q;
+      });
+      const double r = __builtin_fma(i64_to_f64(Cq[KQ - 1]), BETA_D, i64_to_f64(Cq[KQ - 2])) * invs * 0x1p-27;
+      const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
+      if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
+      wsync();
+#pragma unroll
+      for (int i = 0; i < KQ; i++) sd[B3 + g.gl * KQ + i] = -nTq[i] / 2;
+      if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
+      wsync();
+    } else {
+      const double r = __builtin_fma(i64_to_f64(Ch[KB - 1]), BETA_D, i64_to_f64(Ch[KB - 2])) * invs * 0x1p-27;
+      const bool unsure = !(r > 0x1p-16 && r < 1.0 - 0x1p-16);
+      if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
+      wsync();
+#pragma unroll
+      for (int i = 0; i < KB; i++) sd[B0 + g.gl * KB + i] = -nTh[i] / 2;
+      if (top) { sd[F] = 0; sd[F + 1] = 0; sd[F + 2] = 0; }
+      wsync();
+    }
   } else {
     /* Phase 1, steps 0 .. HF-1: T fits the frame's top HF slots, and U's slots below them only
      * carry radicand digits upwards, one per step.  These steps run on a half frame of the top
