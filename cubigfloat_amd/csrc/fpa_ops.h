@@ -1162,20 +1162,33 @@ __host__ __device__ constexpr int div_pad_words(int K, int L) {
 struct DivShortShape {
   int KD, ND, S, TA, JA, C1, KB, FB, B, X;
   bool ok;
+  /* three phases (long frames: ND >= 300, or ND >= 150 with lanes of at most 24 limbs; for
+   * shorter frames and for 32-limb lanes of 152 slots the third frame's relayout costs more
+   * than it saves): the whole frame for about S/3 steps, then KB columns until JB, then the
+   * top FC = KC*L slots (BC below them) */
+  bool T3;
+  int JB, KC, FC, BC;
 };
 __host__ __device__ constexpr DivShortShape div_short_shape(int K, int L) {
   DivShortShape d{};
   d.KD = div_digits_per_lane(K);
   d.ND = d.KD * L;
   d.S = div_steps(K * L);
-  d.TA = (d.S + d.KD) / (2 * d.KD);
-  d.JA = d.TA * d.KD;
   d.C1 = d.S - d.ND + 1;
+  const bool t3 = d.ND >= 300 || (d.ND >= 150 && K <= 24);
+  d.TA = t3 ? (d.S + d.KD + d.KD / 2) / (3 * d.KD) : (d.S + d.KD) / (2 * d.KD);
+  d.JA = d.TA * d.KD;
   d.KB = (d.ND - d.JA + 2 + d.C1 + L - 1) / L;
   d.FB = d.KB * L;
   d.B = d.ND - d.FB;
   d.X = d.FB > d.S - d.JA ? d.FB - (d.S - d.JA) : 0;
   d.ok = d.TA >= 1 && d.JA < d.S && d.KB >= 2 && d.KB < d.KD && d.B >= 0 && d.B <= d.JA - 2 - d.C1;
+  d.JB = d.JA + (d.S - d.JA) / 2;
+  d.KC = (d.ND - d.JB + 2 + d.C1 + L - 1) / L;
+  d.FC = d.KC * L;
+  d.BC = d.ND - d.FC;
+  d.T3 = t3 && d.ok && d.KC >= 2 && d.KC < d.KB && d.BC >= d.B && d.BC <= d.JB - 2 - d.C1;
+  if (d.T3 && d.FC > d.X + d.S - d.JB) d.X = d.FC - (d.S - d.JB);
   return d;
 }
 
@@ -1522,19 +1535,56 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
         since += n;
       }
     };
-    for (; j0 + KB <= S; j0 += KB) {
+    constexpr int JE = SH.T3 ? SH.JB : S;   /* end of the second phase */
+    for (; j0 + KB <= JE; j0 += KB) {
       renorm_before(KB);
       repeat<KB>([&](int u) __attribute__((always_inline)) {
         div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
       });
     }
-    renorm_before((S - SH.JA) % KB);
-    repeat<(S - SH.JA) % KB>([&](int u) __attribute__((always_inline)) {
+    renorm_before((JE - SH.JA) % KB);
+    repeat<(JE - SH.JA) % KB>([&](int u) __attribute__((always_inline)) {
       div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
     });
+    j0 = JE;
     /* the end's estimate as in the whole frame; sure: that frame would keep the digits and
      * find a nonzero remainder */
-    const double t = __builtin_fma(i64_to_f64(Cb[KB - 1]), (double)BETA, i64_to_f64(Cb[KB - 2])) * (invs * 0x1p-27);
+    double t;
+    if constexpr (SH.T3) {
+      /* the third phase on the top FC slots, relaid out like the second */
+      constexpr int KC = SH.KC, BC = SH.BC;
+      static_assert(SH.FC <= SH.X + S - SH.JB, "the relayout stays below the quotient digits");
+      int64_t Cc[KC];
+      int32_t ndc[KC];
+      frame_relayout<KB, KC, SB, BC, L>(g, lds, [&](int i) { return (uint32_t)Cb[i]; }, none,
+                                        [&](int i, uint32_t w) { Cc[i] = (int64_t)w; });
+      frame_relayout<KB, KC, SB, BC, L>(g, lds, [&](int i) { return (uint32_t)((uint64_t)Cb[i] >> 32); }, none,
+                                        [&](int i, uint32_t w) { Cc[i] = (int64_t)(((uint64_t)w << 32) | (uint32_t)Cc[i]); });
+      frame_relayout<KB, KC, SB, BC, L>(g, lds, [&](int i) { return (uint32_t)ndb[i]; }, none,
+                                        [&](int i, uint32_t w) { ndc[i] = (int32_t)w; });
+      auto renorm_c = [&](int n) __attribute__((always_inline)) {
+        if constexpr (ND > 255) {
+          if (since + n > 128) {
+            lazy_renorm<KC, L>(g, Cc, top);
+            since = 0;
+          }
+          since += n;
+        }
+      };
+      for (; j0 + KC <= S; j0 += KC) {
+        renorm_c(KC);
+        repeat<KC>([&](int u) __attribute__((always_inline)) {
+          div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, top, invs);
+        });
+      }
+      renorm_c((S - SH.JB) % KC);
+      repeat<(S - SH.JB) % KC>([&](int u) __attribute__((always_inline)) {
+        div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, top, invs);
+      });
+      t = __builtin_fma(i64_to_f64(Cc[KC - 1]), (double)BETA, i64_to_f64(Cc[KC - 2])) * (invs * 0x1p-27);
+    } else {
+      t = __builtin_fma(i64_to_f64(Cb[KB - 1]), (double)BETA, i64_to_f64(Cb[KB - 2])) * (invs * 0x1p-27);
+    }
     const bool unsure = !(t > 0x1p-16 && t < 1.0 - 0x1p-16);
     if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
   } else {
