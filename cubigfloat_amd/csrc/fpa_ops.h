@@ -1744,15 +1744,14 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  * read from them.  The reference's root (fpa_warp_sqrt.h:195-254) runs 32-bit digits
  * with exact remainders instead.
  */
-/* Digit step k of the lazy sqrt on a frame of KC columns per lane (the whole frame, or its
+/* One digit step of the lazy sqrt on a frame of KC columns per lane (the whole frame, or its
  * top part).  U <- U * beta (kFill: 'fill', nonzero only in the group's lane 0, enters the
  * bottom slot), digit q from the top two columns, U -= q * T, the owner lane's slot idx of
  * T (zero so far) becomes 2q (kOwn: the step's slot lies in the frame).  Returns q. */
 template <int KC, int L, bool kFill, bool kOwn = true>
-__device__ __forceinline__ int32_t sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int k,
-                                               int idx, uint32_t ownm, int kind, bool top, uint32_t nz0,
-                                               uint32_t fill, int32_t &q0, double &invs) {
-  constexpr uint32_t MASK = (1u << DIV_RADIX) - 1;
+__device__ __forceinline__ int32_t sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int idx,
+                                                  uint32_t ownm, int kind, bool top, uint32_t nz0, uint32_t fill,
+                                                  int32_t &q0, double &invs) {
   constexpr double BETA_D = 134217728.0;   /* 2^27 */
   /* U <- U * beta */
   lazy_shift_up<KC, L>(g, C, nz0, top);
@@ -1904,7 +1903,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Cq, KQ);
         repeat<KQ>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, u, KQ - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+          sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
                                       q0, invs);
         });
         for (int tr = 1; tr < L; tr++) {
@@ -1913,7 +1912,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
           asm volatile("" : "+v"(ownm));
           renorm_before(Cq, KQ);
           repeat<KQ>([&](int u) __attribute__((always_inline)) {
-            sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, tr * KQ + u, KQ - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+            sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
           });
         }
         /* into the frame of the top FB slots: below B1 the radicand digits not yet entered
@@ -1949,7 +1948,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KB);
         repeat<KB>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, u, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
                                       q0, invs);
         });
         tr0 = 1;
@@ -1960,7 +1959,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KB);
         repeat<KB>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, tr * KB + u, KB - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
         });
       }
     }
@@ -1969,13 +1968,13 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     for (; k0 + KB <= K2; k0 += KB) {
       renorm_before(Ch, KB);
       repeat<KB>([&](int u) __attribute__((always_inline)) {
-        const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, top, nz0, 0u, q0, invs);
         sd[F - 1 - k0 - u] = q;
       });
     }
     renorm_before(Ch, (K2 - FB) % KB);
     repeat<(K2 - FB) % KB>([&](int u) __attribute__((always_inline)) {
-      const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, k0 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+      const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, top, nz0, 0u, q0, invs);
       sd[F - 1 - k0 - u] = q;
     });
     if constexpr (kQ) {
@@ -2006,13 +2005,13 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       for (; k3 + KQ <= F; k3 += KQ) {
         renorm_before(Cq, KQ);
         repeat<KQ>([&](int u) __attribute__((always_inline)) {
-          const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, k3 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+          const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, top, nz0, 0u, q0, invs);
           sd[F - 1 - k3 - u] = q;
         });
       }
       renorm_before(Cq, (F - K2) % KQ);
       repeat<(F - K2) % KQ>([&](int u) __attribute__((always_inline)) {
-        const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, k3 + u, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, top, nz0, 0u, q0, invs);
         sd[F - 1 - k3 - u] = q;
       });
       const double r = __builtin_fma(i64_to_f64(Cq[KQ - 1]), BETA_D, i64_to_f64(Cq[KQ - 2])) * invs * 0x1p-27;
@@ -2068,7 +2067,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       asm volatile("" : "+v"(ownm));
       renorm_before(Ch, KH);
       repeat<KH>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KH, L, true>(g, Ch, nTh, u, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+        sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
                                     q0, invs);
       });
       for (int tr = 1; tr < L; tr++) {
@@ -2077,7 +2076,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KH);
         repeat<KH>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KH, L, true>(g, Ch, nTh, tr * KH + u, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+          sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
         });
       }
     }
@@ -2117,15 +2116,14 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       asm volatile("" : "+v"(ownm));
       renorm_before(C, RT);
       repeat<RT>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KD, L, false>(g, C, nT, HF + u, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+        sqrt_lazy_step<KD, L, false>(g, C, nT, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
       });
       for (int tr = LT - 1; tr >= 0; tr--) {
         ownm = g.gl == tr ? ~0u : 0u;
         asm volatile("" : "+v"(ownm));
-        const int k0 = HF + RT + (LT - 1 - tr) * KD;
         renorm_before(C, KD);
         repeat<KD>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KD, L, false>(g, C, nT, k0 + u, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+          sqrt_lazy_step<KD, L, false>(g, C, nT, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
         });
       }
     }
