@@ -1768,20 +1768,20 @@ __device__ __forceinline__ int32_t sqrt_lazy_step(const Group<L> &g, int64_t (&C
   }
   q = (int32_t)g.bcast_top((uint32_t)q);
   /* slot idx is still zero in the owner lane (ownm all ones; the lanes above it hold
-   * earlier digits there): the owner sets it to -q for the multiply-adds, so they subtract
-   * q^2 there too, then adds -q again for -2q (no branch on the owner lane) */
-  const uint32_t t = (uint32_t)(-q) & ownm;
+   * earlier digits there, and take 0): the owner sets it to -q for the multiply-adds, so
+   * they subtract q^2 there too, then subtracts q again for -2q (no branch on the owner) */
+  const int32_t qm = (int32_t)((uint32_t)q & ownm);
   if constexpr (kOwn) {
 #pragma unroll
     for (int i = 0; i < KC; i++)
-      if (i == idx) nT[i] = (int32_t)((uint32_t)nT[i] | t);
+      if (i == idx) nT[i] -= qm;
   }
 #pragma unroll
   for (int i = 0; i < KC; i++) C[i] = (int64_t)q * nT[i] + C[i];
   if constexpr (kOwn) {
 #pragma unroll
     for (int i = 0; i < KC; i++)
-      if (i == idx) nT[i] += (int32_t)t;
+      if (i == idx) nT[i] -= qm;
   }
   if (kind == 0) q0 = q;
   if (kind == 1) {
