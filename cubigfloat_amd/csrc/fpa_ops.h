@@ -1270,6 +1270,24 @@ __device__ __forceinline__ void lazy_shift_up(const Group<L> &g, int64_t (&C)[KC
   C[KC - 1] = (int64_t)((uint64_t)C[KC - 1] + (top ? so << DIV_RADIX : 0u));
 }
 
+/* lazy_shift_up with the fold as one v_mad_u64_u32 of the low word (ftop = 2^27 in the top
+ * lane, 0 elsewhere) and a shifted add of the high word's low 5 bits (mtop: all ones in the
+ * top lane), three VALU instead of a 64-bit shift, two selects and an add; both lane
+ * constants opaque, so that they stay registers */
+template <int KC, int L>
+__device__ __forceinline__ void lazy_shift_up(const Group<L> &g, int64_t (&C)[KC], uint32_t nz0, uint32_t ftop,
+                                              uint32_t mtop) {
+  const uint64_t so = (uint64_t)C[KC - 1];
+  const uint32_t lo = g.from_prev_row_and((uint32_t)so, nz0);
+  const uint32_t hi = g.from_prev_row_and((uint32_t)(so >> 32), nz0);
+#pragma unroll
+  for (int i = KC - 1; i > 0; i--) C[i] = C[i - 1];
+  C[0] = (int64_t)(((uint64_t)hi << 32) | lo);
+  const uint64_t t = (uint64_t)(uint32_t)so * ftop + (uint64_t)C[KC - 1];
+  const uint32_t th = (uint32_t)(t >> 32) + (((uint32_t)(so >> 32) & mtop) << DIV_RADIX);
+  C[KC - 1] = (int64_t)(((uint64_t)th << 32) | (uint32_t)t);
+}
+
 /* One carry level over a lazy frame, in place from the top down (column i takes column
  * i-1's carry before that column is reduced): every column but the top lane's top one keeps
  * its low 27 bits and takes the carry of the column below (zero into group lane 0's bottom).
@@ -1750,11 +1768,11 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  * T (zero so far) becomes 2q (kOwn: the step's slot lies in the frame).  Returns q. */
 template <int KC, int L, bool kFill, bool kOwn = true>
 __device__ __forceinline__ int32_t sqrt_lazy_step(const Group<L> &g, int64_t (&C)[KC], int32_t (&nT)[KC], int idx,
-                                                  uint32_t ownm, int kind, bool top, uint32_t nz0, uint32_t fill,
-                                                  int32_t &q0, double &invs) {
+                                                  uint32_t ownm, int kind, uint32_t ftop, uint32_t mtop, uint32_t nz0,
+                                                  uint32_t fill, int32_t &q0, double &invs) {
   constexpr double BETA_D = 134217728.0;   /* 2^27 */
   /* U <- U * beta */
-  lazy_shift_up<KC, L>(g, C, nz0, top);
+  lazy_shift_up<KC, L>(g, C, nz0, ftop, mtop);
   if constexpr (kFill) C[0] = (int64_t)((uint64_t)C[0] | fill);
   int32_t q;
   if (kind == 0) {
@@ -1824,7 +1842,8 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
   static_assert(KH >= 2, "the first trip holds the two special steps");
   const bool top = g.gl == L - 1;
   uint32_t nz0 = g.gl == 0 ? 0u : ~0u;     /* opaque: the column moves stay ANDs */
-  asm volatile("" : "+v"(nz0));
+  uint32_t mtop = top ? ~0u : 0u, ftop = mtop & (1u << DIV_RADIX);
+  asm volatile("" : "+v"(nz0), "+v"(mtop), "+v"(ftop));
   double invs = 0.0;                       /* 2^27 / (T_{F-1} beta + T_{F-2}), from step 2 on */
   int32_t q0 = 0;
   /* |q * 2q_j| < 2^57: one carry level at most every 64 steps keeps the columns below 2^63;
@@ -1903,7 +1922,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Cq, KQ);
         repeat<KQ>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+          sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, ftop, mtop, nz0, fl[u],
                                       q0, invs);
         });
         for (int tr = 1; tr < L; tr++) {
@@ -1912,7 +1931,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
           asm volatile("" : "+v"(ownm));
           renorm_before(Cq, KQ);
           repeat<KQ>([&](int u) __attribute__((always_inline)) {
-            sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+            sqrt_lazy_step<KQ, L, true>(g, Cq, nTq, KQ - 1 - u, ownm, 2, ftop, mtop, nz0, fl[u], q0, invs);
           });
         }
         /* into the frame of the top FB slots: below B1 the radicand digits not yet entered
@@ -1948,7 +1967,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KB);
         repeat<KB>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, ftop, mtop, nz0, fl[u],
                                       q0, invs);
         });
         tr0 = 1;
@@ -1959,7 +1978,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KB);
         repeat<KB>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+          sqrt_lazy_step<KB, L, true>(g, Ch, nTh, KB - 1 - u, ownm, 2, ftop, mtop, nz0, fl[u], q0, invs);
         });
       }
     }
@@ -1968,13 +1987,13 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
     for (; k0 + KB <= K2; k0 += KB) {
       renorm_before(Ch, KB);
       repeat<KB>([&](int u) __attribute__((always_inline)) {
-        const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, ftop, mtop, nz0, 0u, q0, invs);
         sd[F - 1 - k0 - u] = q;
       });
     }
     renorm_before(Ch, (K2 - FB) % KB);
     repeat<(K2 - FB) % KB>([&](int u) __attribute__((always_inline)) {
-      const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, top, nz0, 0u, q0, invs);
+      const int32_t q = sqrt_lazy_step<KB, L, false, false>(g, Ch, nTh, 0, 0u, 2, ftop, mtop, nz0, 0u, q0, invs);
       sd[F - 1 - k0 - u] = q;
     });
     if constexpr (kQ) {
@@ -2005,13 +2024,13 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       for (; k3 + KQ <= F; k3 += KQ) {
         renorm_before(Cq, KQ);
         repeat<KQ>([&](int u) __attribute__((always_inline)) {
-          const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, top, nz0, 0u, q0, invs);
+          const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, ftop, mtop, nz0, 0u, q0, invs);
           sd[F - 1 - k3 - u] = q;
         });
       }
       renorm_before(Cq, (F - K2) % KQ);
       repeat<(F - K2) % KQ>([&](int u) __attribute__((always_inline)) {
-        const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, top, nz0, 0u, q0, invs);
+        const int32_t q = sqrt_lazy_step<KQ, L, false, false>(g, Cq, nTq, 0, 0u, 2, ftop, mtop, nz0, 0u, q0, invs);
         sd[F - 1 - k3 - u] = q;
       });
       const double r = __builtin_fma(i64_to_f64(Cq[KQ - 1]), BETA_D, i64_to_f64(Cq[KQ - 2])) * invs * 0x1p-27;
@@ -2067,7 +2086,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       asm volatile("" : "+v"(ownm));
       renorm_before(Ch, KH);
       repeat<KH>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, top, nz0, fl[u],
+        sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, u == 0 ? 0 : u == 1 ? 1 : 2, ftop, mtop, nz0, fl[u],
                                     q0, invs);
       });
       for (int tr = 1; tr < L; tr++) {
@@ -2076,7 +2095,7 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
         asm volatile("" : "+v"(ownm));
         renorm_before(Ch, KH);
         repeat<KH>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, 2, top, nz0, fl[u], q0, invs);
+          sqrt_lazy_step<KH, L, true>(g, Ch, nTh, KH - 1 - u, ownm, 2, ftop, mtop, nz0, fl[u], q0, invs);
         });
       }
     }
@@ -2116,14 +2135,14 @@ __device__ __forceinline__ bool core_sqrt_lazy(const Group<L> &g, uint32_t *lds,
       asm volatile("" : "+v"(ownm));
       renorm_before(C, RT);
       repeat<RT>([&](int u) __attribute__((always_inline)) {
-        sqrt_lazy_step<KD, L, false>(g, C, nT, RT - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+        sqrt_lazy_step<KD, L, false>(g, C, nT, RT - 1 - u, ownm, 2, ftop, mtop, nz0, 0u, q0, invs);
       });
       for (int tr = LT - 1; tr >= 0; tr--) {
         ownm = g.gl == tr ? ~0u : 0u;
         asm volatile("" : "+v"(ownm));
         renorm_before(C, KD);
         repeat<KD>([&](int u) __attribute__((always_inline)) {
-          sqrt_lazy_step<KD, L, false>(g, C, nT, KD - 1 - u, ownm, 2, top, nz0, 0u, q0, invs);
+          sqrt_lazy_step<KD, L, false>(g, C, nT, KD - 1 - u, ownm, 2, ftop, mtop, nz0, 0u, q0, invs);
         });
       }
     }
