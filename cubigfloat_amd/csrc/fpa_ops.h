@@ -1445,8 +1445,8 @@ __device__ __forceinline__ int64_t normalize_exact(const Group<L> &g, int64_t (&
  * digit from the top two columns (top lane) broadcast, R -= q * D, q to *qd */
 template <int KC, int L>
 __device__ __forceinline__ void div_lazy_step(const Group<L> &g, int64_t (&C)[KC], const int32_t (&nd)[KC], int32_t *qd,
-                                              uint32_t nz0, bool top, double invs) {
-  lazy_shift_up<KC, L>(g, C, nz0, top);
+                                              uint32_t nz0, uint32_t ftop, uint32_t mtop, double invs) {
+  lazy_shift_up<KC, L>(g, C, nz0, ftop, mtop);
   const int32_t q = (int32_t)g.bcast_top((uint32_t)lazy_top_digit(C[KC - 1], C[KC - 2], invs));
 #pragma unroll
   for (int i = 0; i < KC; i++) C[i] = (int64_t)q * nd[i] + C[i];
@@ -1509,8 +1509,8 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
   }
   const double invs = inv * 0x1p27;
   const bool top = g.gl == L - 1;
-  uint32_t nz0 = g.gl == 0 ? 0u : ~0u;
-  asm volatile("" : "+v"(nz0));
+  uint32_t nz0 = g.gl == 0 ? 0u : ~0u, mtop = top ? ~0u : 0u, ftop = mtop & (1u << DIV_RADIX);
+  asm volatile("" : "+v"(nz0), "+v"(mtop), "+v"(ftop));
   constexpr DivShortShape SH = div_short_shape(K, L);
   static_assert(!kShort || SH.ok, "short division shape");
   /* quotient digit of step j at S-1-j (X words up in the short division) */
@@ -1526,7 +1526,7 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
       if (j0 > 0 && (j0 / KD) % RT == 0) lazy_renorm<KD, L>(g, C, top);
     }
     repeat<KD>([&](int u) __attribute__((always_inline)) {
-      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, top, invs);
+      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
     });
   }
   if constexpr (kShort) {
@@ -1557,12 +1557,12 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
     for (; j0 + KB <= JE; j0 += KB) {
       renorm_before(KB);
       repeat<KB>([&](int u) __attribute__((always_inline)) {
-        div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
+        div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
       });
     }
     renorm_before((JE - SH.JA) % KB);
     repeat<(JE - SH.JA) % KB>([&](int u) __attribute__((always_inline)) {
-      div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, top, invs);
+      div_lazy_step<KB, L>(g, Cb, ndb, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
     });
     j0 = JE;
     /* the end's estimate as in the whole frame; sure: that frame would keep the digits and
@@ -1592,12 +1592,12 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
       for (; j0 + KC <= S; j0 += KC) {
         renorm_c(KC);
         repeat<KC>([&](int u) __attribute__((always_inline)) {
-          div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, top, invs);
+          div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
         });
       }
       renorm_c((S - SH.JB) % KC);
       repeat<(S - SH.JB) % KC>([&](int u) __attribute__((always_inline)) {
-        div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, top, invs);
+        div_lazy_step<KC, L>(g, Cc, ndc, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
       });
       t = __builtin_fma(i64_to_f64(Cc[KC - 1]), (double)BETA, i64_to_f64(Cc[KC - 2])) * (invs * 0x1p-27);
     } else {
@@ -1607,7 +1607,7 @@ __device__ __forceinline__ bool core_div_lazy(const Group<L> &g, uint32_t *lds, 
     if (__builtin_expect(__ballot(top && unsure) != 0, 0)) return false;
   } else {
     repeat<S % KD>([&](int u) __attribute__((always_inline)) {
-      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, top, invs);
+      div_lazy_step<KD, L>(g, C, nd, qd + (S - 1 - j0 - u), nz0, ftop, mtop, invs);
     });
   }
   uint32_t rem_nz = 1;
