@@ -12,16 +12,20 @@
 
 /* Occupancy target (waves per SIMD) of an element-wise kernel.  sqrt with 16 limbs on 8
  * lanes needs 130-140 VGPRs left alone (3 waves per SIMD); held to 128 it spills a little
- * outside the digit loop and runs faster at 4.  div with 32 limbs on 2 or 4 lanes needs
- * about 172 VGPRs; held to 168 (3 waves per SIMD) it spills a few words outside the loop
- * and runs 3-6% faster (with 8 or 16 lanes the same target spilled inside the loop and
- * ran 11x slower).  Others take the compiler's own choice (1 = no target).  A build can set
- * one target for every kernel (make EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
+ * outside the digit loop and runs faster at 4.  div with 32 limbs on 4 lanes needs about
+ * 172 VGPRs; held to 168 (3 waves per SIMD) it spills a few words outside the loop and runs
+ * 3-6% faster (with 8 or 16 lanes the same target spilled inside the loop and ran 11x
+ * slower).  On 2 lanes the same target gave wrong low words in a few percent of the values
+ * of large launches, differently from run to run (and none in small ones), so that shape
+ * takes the compiler's own choice; the 4-lane one gives none on the same launches (a GPU
+ * test runs every div and sqrt shape on 2^16 values).  Others take the compiler's own
+ * choice (1 = no target).  A build can set one target for every kernel (make
+ * EXTRA_HIPFLAGS=-DFPA_WAVES_PER_EU=n). */
 __host__ __device__ constexpr int fpa_waves_per_eu(int op, int K, int L) {
 #ifdef FPA_WAVES_PER_EU
   return (void)op, (void)K, (void)L, FPA_WAVES_PER_EU;
 #else
-  return op == 5 && K == 16 && L == 8 ? 4 : op == 4 && K == 32 && L <= 4 ? 3 : 1;
+  return op == 5 && K == 16 && L == 8 ? 4 : op == 4 && K == 32 && L == 4 ? 3 : 1;
 #endif
 }
 #define FPA_KERNEL_ATTRS(OP, K, L) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fpa_waves_per_eu(OP, K, L))))
