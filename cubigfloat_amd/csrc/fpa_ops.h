@@ -1694,9 +1694,10 @@ __device__ __forceinline__ void op_div(const fpa_arguments &a, const Slot &s, co
   load_value<K, L>(g, a.b_limbs, s, B);
   if constexpr (div_short_shape(K, L).ok) {
     if (!core_div_lazy<K, L, true>(g, lds, (int)a.mode, a.a_exps[s.v], A, a.b_exps[s.v], B, re, R)) {
-      /* some group of the wave was left unsure: the whole wave runs the whole frame, from
-       * the operands reloaded (the clobber keeps the compiler from holding the first loads'
-       * registers through the short run instead) */
+      /* some group of the wave was left unsure: every group of the wave but the special
+       * classes (already done) runs the whole frame, from the operands reloaded (the clobber
+       * keeps the compiler from holding the first loads' registers through the short run
+       * instead) */
       asm volatile("" ::: "memory");
       load_value<K, L>(g, a.a_limbs, s, A);
       load_value<K, L>(g, a.b_limbs, s, B);
@@ -1719,9 +1720,10 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
   load_value<K, L>(g, a.a_limbs, s, A);
   if constexpr (sqrt_short_ok<K, L>()) {
     if (!core_sqrt_lazy<K, L, true>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R)) {
-      /* the wave's short frames left a root undecided: the whole wave runs the full frame,
-       * from the operand reloaded (the clobber keeps the compiler from holding the first
-       * load's registers through the short run instead) */
+      /* the short frames left some group's root undecided: every group of the wave but the
+       * special classes (already done) runs the full frame, from the operand reloaded (the
+       * clobber keeps the compiler from holding the first load's registers through the
+       * short run instead) */
       asm volatile("" ::: "memory");
       load_value<K, L>(g, a.a_limbs, s, A);
       core_sqrt_lazy<K, L, false>(g, lds, (int)a.mode, a.a_exps[s.v], A, re, R);
@@ -1749,8 +1751,12 @@ __device__ __forceinline__ void op_sqrt(const fpa_arguments &a, const Slot &s, c
  *      lower digits are below 2^-53 of it), V from the top two columns;
  *   3. U -= q * T (one v_mad_i64_i32 per column), the owner of slot pos subtracts q^2 and
  *      sets T's slot to 2q.
- * In the first HF = ceil(KD/2)*L steps T fits the frame's top HF slots and the slots below
- * only move radicand digits upwards: those steps run on a half frame of the top HF slots
+ The short root (kShort) runs all F steps on the top (F+2)/2 slots, since no later digit
+ * and not the end's estimate reads a slot below k - 1 after step k (long roots: on the top
+ * quarter for the first and last quarter of the steps); where its end is unsure for some
+ * group, the wave's groups run the full frame instead.  The full frame: in the first
+ * HF = ceil(KD/2)*L steps T fits the frame's top HF slots and the slots below only move
+ * radicand digits upwards, so those steps run on a half frame of the top HF slots
  * (ceil(KD/2) columns per lane), the radicand digits entering at its bottom one per step,
  * and the frame is then laid out whole again through LDS (about 3/4 of the multiply-adds
  * of F whole-frame steps).
